@@ -1,0 +1,206 @@
+/*
+ * kpolrad.h -- C ABI of the MI355X photon-packet transport engine.
+ *
+ * Drop-in for the photon module of kratos_polrad
+ * (src/modules/particle/radiation/).  Every entry point takes plain pointers
+ * and sizes, never throws, and returns a kp_status; kp_last_error() gives the
+ * message of the last failure on the calling thread.  Device pointers are
+ * hipMalloc'ed memory owned by the caller unless stated otherwise; the
+ * stream argument is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference src/):
+ *   mesh tree + face-neighbour table ... mesh/tree/tree.cpp:177-218,
+ *                                        modules/particle/types/proxy.h:22-62,
+ *                                        modules/particle/types/block_map.h:225-275
+ *   block geometry ..................... mesh/meshgen/meshgen.cpp:286-351,378-432
+ *   per-cell radiation storage ......... modules/particle/radiation/rad_data.cpp:13-25
+ *   one particle step (generate +
+ *     integrate + comm round) .......... modules/particle/driver.h:59-79,
+ *                                        evolve/generate_par.h:51-62,
+ *                                        evolve/integrator.h:47-106,
+ *                                        radiation/photon.h:20-125,
+ *                                        types/proxy.h:64-121
+ *   pool compaction / send gather ...... types/particle_pool.h:191-248,
+ *                                        boundary/comm.h:118-127,226-239
+ */
+#ifndef KPOLRAD_H
+#define KPOLRAD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kp_status {
+    KP_OK = 0,
+    KP_ERR_ARG = 1,         /* invalid argument (std::invalid_argument)      */
+    KP_ERR_HIP = 2,         /* HIP runtime error (reference hip_api throws)   */
+    KP_ERR_ALLOC = 3,       /* allocation failure                             */
+    KP_ERR_STATE = 4,       /* call order violated (e.g. no media set)        */
+    KP_ERR_UNSUPPORTED = 5  /* valid request this build does not implement    */
+} kp_status;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *kp_last_error(void);
+
+/* ================================================================== */
+/* Host mesh: block tree with 2:1 balance, ids, geometry, face table.  */
+/* ================================================================== */
+
+typedef struct kp_mesh kp_mesh;
+
+/* Block-table view of a mesh (arrays owned by the kp_mesh).  Block b has
+ * cells i in [0,n_cell)^3; its cell (i0,i1,i2) has linear index
+ *   b*n_cell^3 + i0 + n_cell*(i1 + n_cell*i2)
+ * (the dat_3d_t order of mesh/types/dat_3d.h:145-147 without ghosts).
+ * nb_id[b][a][s][j0][j1] is the face neighbour across side s of axis a;
+ * j0/j1 pick the sub-face along axes (a+1)%3/(a+2)%3 when the neighbour is
+ * finer (proxy.h:11-20 neighbors[3][2][2][2]).  nb_shift bit j is shift[j]
+ * (which half of a coarser neighbour's face this block touches). */
+typedef struct kp_mesh_desc {
+    int32_t n_blocks;
+    int32_t n_cell;        /* cells per block edge, even (n_cell_block)    */
+    int32_t periodic[3];
+    double x_min[3];
+    double x_max[3];
+    const double *xf0;     /* [n_blocks][3] first face coordinate          */
+    const double *dx0;     /* [n_blocks][3] cell width                     */
+    const double *vol0;    /* [n_blocks]    cell volume                    */
+    const int32_t *level;  /* [n_blocks]                                   */
+    const int32_t *logic;  /* [n_blocks][3] logical block index at level   */
+    const int32_t *nb_id;  /* [n_blocks][24] neighbour block id or -1      */
+    const uint8_t *nb_shift; /* [n_blocks][24]                             */
+    const int8_t *d_lvl;   /* [n_blocks][6]  neighbour level - own level   */
+    const uint8_t *phybnd; /* [n_blocks] bit 2a+s: physical boundary       */
+} kp_mesh_desc;
+
+/* meshgen::base_t::read/init (meshgen.cpp:13-106): root grid of
+ * root_blocks[a] blocks of n_cell^3 cells on [x_min,x_max]. */
+kp_status kp_mesh_create(const int32_t root_blocks[3], int32_t n_cell,
+                         const double x_min[3], const double x_max[3],
+                         const int32_t periodic[3], kp_mesh **out);
+void kp_mesh_destroy(kp_mesh *mesh);
+
+/* tree_t::insert (tree.cpp:202-218) of n logical regions at `level`
+ * (idx is [n][3]); parents are inserted and masked, 2:1 balance kept. */
+kp_status kp_mesh_refine(kp_mesh *mesh, int32_t level, const int32_t *idx,
+                         int64_t n);
+
+/* load_refinement SMR box (meshgen.cpp:111-142): every block at `level`
+ * whose index range covers [x_lo, x_hi] (search_blk_idx, meshgen.cpp:353-373). */
+kp_status kp_mesh_refine_region(kp_mesh *mesh, int32_t level, const double x_lo[3],
+                                const double x_hi[3]);
+
+/* fill_base_level + dist_default ids + neighbours + geometry
+ * (meshgen.cpp:144-188, 286-351, 378-432). */
+kp_status kp_mesh_finalize(kp_mesh *mesh);
+
+kp_status kp_mesh_describe(const kp_mesh *mesh, kp_mesh_desc *desc);
+
+/* Leaf block and cell containing x (cells are half-open [x_l,x_r)),
+ * geo_t::idx semantics (geometry.h:120-140).  ib = -1 if outside. */
+kp_status kp_mesh_locate(const kp_mesh *mesh, const double x[3],
+                         int32_t *ib, int32_t i[3]);
+
+/* ================================================================== */
+/* Transport engine                                                    */
+/* ================================================================== */
+
+typedef struct kp_engine kp_engine;
+
+/* Copies the block table to the device (block_map_t::update,
+ * block_map.h:225-275).  The engine is bound to `device`. */
+kp_status kp_engine_create(const kp_mesh_desc *mesh, int32_t device,
+                           kp_engine **out);
+void kp_engine_destroy(kp_engine *eng);
+
+/* Per-cell medium, host arrays of n_blocks*n_cell^3 floats in the block
+ * table's cell order: kappa_rho (extinction per unit length, the rad_t
+ * mfp_i field of rad_data.h:16), albedo, HG asymmetry g (may be NULL:
+ * then 0).  Copied to the device. */
+kp_status kp_set_media(kp_engine *eng, const float *kappa_rho,
+                       const float *albedo, const float *g);
+
+enum { KP_THOMSON = 0, KP_HG_RAYLEIGH = 1 };
+enum { KP_F64 = 0, KP_F32 = 1 };
+enum { KP_BAND_CELL_ALBEDO = 1, KP_BAND_CELL_G = 2 };
+
+/* One wavelength (or source set).  Band b uses Philox stream b and tally
+ * component b (rad_t::flx has n_hv components, radiation.h:24-35). */
+typedef struct kp_band {
+    double kappa_scale;  /* multiplies kappa_rho                           */
+    double albedo;       /* used unless flags & KP_BAND_CELL_ALBEDO        */
+    double g;            /* used unless flags & KP_BAND_CELL_G             */
+    uint64_t n_packets;  /* packets of this band in this call              */
+    uint32_t flags;
+    uint32_t pad_;
+} kp_band;
+
+/* Isotropic, unpolarized point source. */
+typedef struct kp_source {
+    double x[3];
+    double weight;       /* Stokes I of each emitted packet                */
+} kp_source;
+
+typedef struct kp_run {
+    uint32_t seed;           /* Philox key word 0                          */
+    int32_t scatter_model;   /* KP_THOMSON | KP_HG_RAYLEIGH                */
+    double p_max;            /* peak linear polarization (HG-Rayleigh)     */
+    int32_t n_bands;
+    int32_t precision;       /* KP_F64: FP64 global coordinates;
+                                KP_F32: FP32 block-local transport,
+                                FP64 optical depth and tallies            */
+    const kp_band *bands;    /* host array [n_bands]                       */
+    uint64_t packet_offset;  /* id of the first packet of every band       */
+    int32_t step_max;        /* march-step budget ([particle] step_max)    */
+    int32_t n_incl;          /* observers (peel-off images)                */
+    const double *incl_deg;  /* host [n_incl], azimuth 0                   */
+    double rr_weight;        /* Russian roulette when I < rr_weight*weight */
+    double rr_survive;       /* survival probability                       */
+    int32_t image_n;         /* pixels per image side                      */
+    int32_t pad_;
+    double image_half;       /* half-width of the field of view            */
+} kp_run;
+
+/* Counters accumulated by kp_forward (uint64 each). */
+enum {
+    KP_C_CROSS_MAIN = 0,  /* march steps of packet paths                   */
+    KP_C_CROSS_PEEL = 1,  /* march steps of peel-off rays                  */
+    KP_C_EVENTS = 2,      /* interactions                                  */
+    KP_C_ESCAPED = 3,
+    KP_C_KILLED_RR = 4,
+    KP_C_KILLED_STEPS = 5,
+    KP_C_LOST = 6,        /* left through a face with no neighbour, or stuck */
+    KP_C_PACKETS = 7,
+    KP_NCOUNTERS = 8
+};
+
+/* Per-packet end record (doubles) written when end_state != NULL. */
+enum {
+    KP_ES_X = 0, KP_ES_D = 3, KP_ES_I = 6, KP_ES_Q = 7, KP_ES_U = 8,
+    KP_ES_EVENTS = 9, KP_ES_CROSS = 10, KP_ES_STATUS = 11, KP_ES_IB = 12,
+    KP_ES_CELL = 13, KP_ES_STRIDE = 16
+};
+enum { KP_END_ESCAPED = 1, KP_END_RR = 2, KP_END_STEPS = 3, KP_END_LOST = 4 };
+
+/* Device outputs; each may be NULL.  kp_forward ADDS into J, image and
+ * counters (chunked runs accumulate); end_state is overwritten. */
+typedef struct kp_tallies {
+    double *J;          /* [n_cells][n_bands]: sum of I*l/V (photon.h:73-83) */
+    double *image;      /* [n_bands][n_incl][image_n][image_n][3] I,Q,U    */
+    uint64_t *counters; /* [KP_NCOUNTERS]                                  */
+    double *end_state;  /* [sum of n_packets][KP_ES_STRIDE], band-major    */
+} kp_tallies;
+
+/* Emit, march, scatter and peel off every packet of every band in one
+ * launch on `stream` (asynchronous). */
+kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,
+                     kp_tallies *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KPOLRAD_H */

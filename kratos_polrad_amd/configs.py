@@ -1,0 +1,144 @@
+"""Synthetic workloads (meshes + media) for the photon transport benchmarks.
+
+Config 1 (the reference's CPU-runnable case): uniform 64^3 unit cube as
+4x4x4 blocks of 16^3, a homogeneous Thomson sphere of radius 0.4 with radial
+optical depth 1 (kappa_rho = 2.5 inside, cell-centre test), albedo 1, an
+isotropic unpolarized point source at the centre, seed 42, a 128x128 I/Q/U
+peel-off image at inclination 90 degrees.
+
+Config 2: 3-level block AMR, rho ~ r^-2 times a lognormal field (sigma 0.5,
+seeded Gaussian random field with a k^-11/3 power spectrum), refinement of
+every block whose max |grad rho| dx / rho exceeds 0.5, HG g=0.6, albedo 0.5,
+Rayleigh-form linear polarization with p_max 0.5.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .engine import BandSpec, RunSpec
+from .mesh import Mesh
+
+
+@dataclass
+class Workload:
+    name: str
+    mesh: Mesh
+    kappa: np.ndarray
+    albedo: np.ndarray
+    g: np.ndarray
+    source: tuple
+    run: RunSpec
+
+
+def config1(n_packets: int = 1_000_000, precision: int = _abi.KP_F64, seed: int = 42,
+            n_cell: int = 16, root: int = 4, image_n: int = 128) -> Workload:
+    mesh = Mesh(root_blocks=(root, root, root), n_cell=n_cell).finalize()
+    xc = mesh.cell_centers()
+    r = np.linalg.norm(xc - 0.5, axis=1)
+    R, tau = 0.4, 1.0
+    kappa = np.where(r < R, tau / R, 0.0).astype(np.float32)
+    albedo = np.ones(mesh.n_cells, dtype=np.float32)
+    g = np.zeros(mesh.n_cells, dtype=np.float32)
+    run = RunSpec(bands=[BandSpec(n_packets=n_packets, albedo=1.0)], seed=seed,
+                  scatter_model=_abi.KP_THOMSON, p_max=1.0, precision=precision,
+                  incl_deg=(90.0,), image_n=image_n, image_half=0.5)
+    return Workload("config1_uniform64_thomson_sphere", mesh, kappa, albedo, g,
+                    (0.5, 0.5, 0.5), run)
+
+
+def gaussian_random_field(n: int, seed: int, slope: float = -11.0 / 3.0) -> np.ndarray:
+    """Unit-variance periodic Gaussian random field on n^3 with P(k) ~ k^slope."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n) * n
+    kz = np.fft.rfftfreq(n) * n
+    kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = np.zeros_like(kk)
+    nz = kk > 0
+    amp[nz] = kk[nz] ** (slope / 2.0)
+    noise = rng.standard_normal((n, n, n // 2 + 1)) + 1j * rng.standard_normal((n, n, n // 2 + 1))
+    f = np.fft.irfftn(noise * amp, s=(n, n, n))
+    f -= f.mean()
+    f /= f.std()
+    return f
+
+
+def _sample_field(field: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Nearest-cell lookup of a unit-cube periodic field at points x [m,3]."""
+    n = field.shape[0]
+    idx = np.clip((x * n).astype(np.int64), 0, n - 1)
+    return field[idx[:, 0], idx[:, 1], idx[:, 2]]
+
+
+def amr_density(root: int = 4, n_cell: int = 16, levels: int = 3, sigma: float = 0.5,
+                seed: int = 7, threshold: float = 0.5, field_n: int = 256,
+                r_soft: float = 0.02, extra_sigma: float = 0.0, extra_seed: int = 11):
+    """Criterion-refined block mesh and the density rho ~ r^-2 * lognormal on it.
+
+    A block at level l is refined when max over its cells of
+    |grad rho| * dx / rho exceeds ``threshold`` (central differences of the
+    analytic density on that block's cell centres), 2:1 balance kept by the
+    tree."""
+    grf = gaussian_random_field(field_n, seed)
+    grf2 = gaussian_random_field(field_n, extra_seed) if extra_sigma > 0 else None
+
+    def rho_at(x):
+        r2 = np.sum((x - 0.5) ** 2, axis=1) + r_soft ** 2
+        ln = sigma * _sample_field(grf, x) - 0.5 * sigma ** 2
+        if grf2 is not None:
+            ln = ln + extra_sigma * _sample_field(grf2, x) - 0.5 * extra_sigma ** 2
+        return np.exp(ln) / r2
+
+    refine = {}
+    # decide refinement level by level on a provisional uniform set of blocks
+    cur = [(0, i, j, k) for i in range(root) for j in range(root) for k in range(root)]
+    for lvl in range(levels - 1):
+        nxt = []
+        for (l, i, j, k) in cur:
+            h = 1.0 / (root << l) / n_cell
+            o = np.array([i, j, k], dtype=np.float64) * n_cell * h
+            c = (np.arange(n_cell) + 0.5) * h
+            gx, gy, gz = np.meshgrid(c, c, c, indexing="ij")
+            pts = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1) + o
+            rho = rho_at(pts).reshape(n_cell, n_cell, n_cell)
+            gr = np.sqrt(sum(np.gradient(rho, axis=a) ** 2 for a in range(3)))
+            if np.max(gr / rho) > threshold:
+                refine.setdefault(l + 1, []).append((i, j, k))
+                for di in range(2):
+                    for dj in range(2):
+                        for dk in range(2):
+                            nxt.append((l + 1, 2 * i + di, 2 * j + dj, 2 * k + dk))
+        cur = nxt
+    mesh = Mesh(root_blocks=(root, root, root), n_cell=n_cell)
+    for lvl in sorted(refine):
+        # refining a block means inserting its children at lvl
+        kids = []
+        for (i, j, k) in refine[lvl]:
+            for di in range(2):
+                for dj in range(2):
+                    for dk in range(2):
+                        kids.append((2 * i + di, 2 * j + dj, 2 * k + dk))
+        mesh.refine(lvl, np.array(kids, dtype=np.int32))
+    mesh.finalize()
+    rho = rho_at(mesh.cell_centers())
+    return mesh, rho
+
+
+def config2(n_packets: int = 100_000_000, precision: int = _abi.KP_F32, seed: int = 42,
+            tau_scale: float = 10.0, **kw) -> Workload:
+    mesh, rho = amr_density(**kw)
+    # normalise so the centre-to-edge optical depth along +x is ~tau_scale
+    xc = mesh.cell_centers()
+    on_axis = (np.abs(xc[:, 1] - 0.5) < 0.02) & (np.abs(xc[:, 2] - 0.5) < 0.02) & (xc[:, 0] > 0.5)
+    dens = rho.copy()
+    lvl_h = np.repeat(mesh.dx0[:, 0], mesh.n3)
+    col = np.sum(dens[on_axis] * lvl_h[on_axis]) / max(1, int(round(0.04 / lvl_h[on_axis].mean()))) ** 2
+    kappa = (dens * (tau_scale / max(col, 1e-30))).astype(np.float32)
+    albedo = np.full(mesh.n_cells, 0.5, dtype=np.float32)
+    g = np.full(mesh.n_cells, 0.6, dtype=np.float32)
+    run = RunSpec(bands=[BandSpec(n_packets=n_packets, albedo=0.5, g=0.6)], seed=seed,
+                  scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
+                  incl_deg=(90.0,), image_n=128, image_half=0.5)
+    return Workload("config2_amr3_r2_lognormal_hg", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)

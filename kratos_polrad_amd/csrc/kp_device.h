@@ -1,0 +1,91 @@
+// kp_device.h -- device-side data views shared by the transport kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kp_math.h"
+
+namespace kp {
+
+constexpr int kMaxIncl = 8;
+constexpr double kTauCut = 40.0; // peel-off rays stop once tau exceeds this
+
+// One band (wavelength / source set) as the kernels see it.
+struct dband {
+    double kappa_scale;
+    double albedo;
+    double g;
+    uint32_t flags;
+    uint32_t pad;
+};
+
+// Everything a transport launch needs, passed by value.
+struct kparams {
+    // mesh (block table, FP64)
+    const double *xf0;
+    const double *dx0;
+    const double *vol0;
+    const int32_t *nb_id;
+    const uint8_t *nb_shift;
+    const int8_t *d_lvl;
+    const uint8_t *phybnd;
+    // FP32 block-local view: per block h[3] (cell width), origin[3], volume
+    const float *h32;
+    const float *org32;
+    const float *vol32;
+    int n_cell;
+    int n3;
+    int periodic[3];
+    double xmin[3], xmax[3];
+    // medium
+    const float *kappa;
+    const float *albedo;
+    const float *g;
+    // bands
+    const dband *bands;
+    const uint64_t *band_start; // [n_bands+1]
+    int n_bands;
+    uint64_t total;
+    uint64_t packet_offset;
+    // source
+    double src_x[3];
+    double w0;
+    int src_ib;
+    int src_i[3];
+    float src_xl[3]; // source position in block-local cell units (FP32 path)
+    // run
+    uint32_t seed;
+    int model;
+    double pmax;
+    int step_max;
+    double rr_w;
+    double rr_s;
+    int n_incl;
+    double no[kMaxIncl][3], ev[kMaxIncl][3], eh[kMaxIncl][3];
+    double center[3];
+    int image_n;
+    double image_half;
+    double pix_scale;
+    // outputs
+    double *J;
+    double *image;
+    unsigned long long *counters;
+    double *end_state;
+};
+
+__device__ __forceinline__ void atomic_add_f64(double *p, double v)
+{
+    unsafeAtomicAdd(p, v); // native global_atomic_add_f64 (coarse-grained memory)
+}
+
+// Band of a global packet slot (band_start is small and cache resident).
+__device__ __forceinline__ int band_of(const kparams &P, uint64_t slot)
+{
+    int b = 0;
+    while (b + 1 < P.n_bands && slot >= P.band_start[b + 1])
+        ++b;
+    return b;
+}
+
+} // namespace kp

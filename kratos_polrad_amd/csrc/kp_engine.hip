@@ -1,0 +1,451 @@
+// kp_engine.hip -- transport engine: device copies of the block table and
+// medium, and the forward launch.
+//
+// One launch replaces the reference's per-cycle orchestration for photons
+// (driver.h:59-79): the generator (generate_par.h:51-62) and the integrator
+// (integrator.h:47-106) are fused -- each thread draws its packet's initial
+// state from Philox keyed by the packet id instead of reading an emitted
+// pool, then runs the whole history (photon.h:104-125 runs the whole march
+// inside one ker_itg thread too).  Bands are the n_hv photon-energy bins of
+// rad_t::flx (radiation.h:24-35).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kp_device.h"
+#include "kp_error.h"
+#include "kp_transport32.h"
+#include "kp_transport64.h"
+
+#define KP_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw kp::error(KP_ERR_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct kp_engine {
+    int device = 0;
+    int n_blocks = 0, n_cell = 0, n3 = 0;
+    int periodic[3] = {0, 0, 0};
+    double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
+    std::vector<double> xf0_h, dx0_h;
+    // device block table
+    double *xf0 = nullptr, *dx0 = nullptr, *vol0 = nullptr;
+    int32_t *nb_id = nullptr;
+    uint8_t *nb_shift = nullptr, *phybnd = nullptr;
+    int8_t *d_lvl = nullptr;
+    float *h32 = nullptr, *org32 = nullptr, *vol32 = nullptr;
+    // medium
+    float *kappa = nullptr, *albedo = nullptr, *g = nullptr;
+    bool media_set = false;
+    // band table (device + pinned staging + host shadow)
+    kp::dband *bands_d = nullptr;
+    uint64_t *band_start_d = nullptr;
+    void *stage = nullptr;
+    int bands_cap = 0;
+    std::vector<unsigned char> shadow;
+    // launch shape
+    int block_threads = 256;
+    int grid_blocks = 0; // 0: one thread per packet
+};
+
+namespace {
+
+template <class T> T *dmalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0)
+        n = 1;
+    KP_HIP(hipMalloc(&p, n * sizeof(T)));
+    return (T *)p;
+}
+
+template <class T> void dfree(T *&p)
+{
+    if (p)
+        (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <class T> void upload(T *dst, const T *src, size_t n)
+{
+    KP_HIP(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct device_guard {
+    int prev = 0;
+    explicit device_guard(int dev)
+    {
+        KP_HIP(hipGetDevice(&prev));
+        if (prev != dev)
+            KP_HIP(hipSetDevice(dev));
+    }
+    ~device_guard() { (void)hipSetDevice(prev); }
+};
+
+// wave-level sum of a 64-bit counter, then one atomic per wave
+__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint64_t v)
+{
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    uint64_t sum = 0;
+    // reduce in 64-bit using two 32-bit shuffles per step
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t olo = __shfl_xor(lo, off, 64);
+        const uint32_t ohi = __shfl_xor(hi, off, 64);
+        const uint64_t s = ((uint64_t)hi << 32 | lo) + ((uint64_t)ohi << 32 | olo);
+        lo = (uint32_t)s;
+        hi = (uint32_t)(s >> 32);
+    }
+    sum = ((uint64_t)hi << 32) | lo;
+    if ((threadIdx.x & 63) == 0 && sum)
+        atomicAdd(dst, (unsigned long long)sum);
+}
+
+template <class C> __device__ __forceinline__ void flush_counters(const kp::kparams &P, const C &c)
+{
+    if (!P.counters)
+        return;
+    wave_add_u64(P.counters + KP_C_CROSS_MAIN, c.cross);
+    wave_add_u64(P.counters + KP_C_CROSS_PEEL, c.peel);
+    wave_add_u64(P.counters + KP_C_EVENTS, c.events);
+    wave_add_u64(P.counters + KP_C_ESCAPED, c.escaped);
+    wave_add_u64(P.counters + KP_C_KILLED_RR, c.rr);
+    wave_add_u64(P.counters + KP_C_KILLED_STEPS, c.steps);
+    wave_add_u64(P.counters + KP_C_LOST, c.lost);
+    wave_add_u64(P.counters + KP_C_PACKETS, c.packets);
+}
+
+__global__ void __launch_bounds__(256) k_forward_f64(kp::kparams P)
+{
+    kp::f64::counters_t C;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
+         slot += stride) {
+        const int band = kp::band_of(P, slot);
+        const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
+        kp::f64::packet(P, band, pid, slot, C);
+    }
+    flush_counters(P, C);
+}
+
+__global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
+{
+    kp::f32::counters_t C;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
+         slot += stride) {
+        const int band = kp::band_of(P, slot);
+        const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
+        kp::f32::packet(P, band, pid, slot, C);
+    }
+    flush_counters(P, C);
+}
+
+void free_all(kp_engine *e)
+{
+    dfree(e->xf0);
+    dfree(e->dx0);
+    dfree(e->vol0);
+    dfree(e->nb_id);
+    dfree(e->nb_shift);
+    dfree(e->phybnd);
+    dfree(e->d_lvl);
+    dfree(e->h32);
+    dfree(e->org32);
+    dfree(e->vol32);
+    dfree(e->kappa);
+    dfree(e->albedo);
+    dfree(e->g);
+    dfree(e->bands_d);
+    dfree(e->band_start_d);
+    if (e->stage)
+        (void)hipHostFree(e->stage);
+    e->stage = nullptr;
+}
+
+// geo_t::idx semantics: leaf block with x in [lo, hi) on every axis
+bool locate(const kp_engine *e, const double x[3], int &ib, int i[3])
+{
+    for (int b = 0; b < e->n_blocks; ++b) {
+        bool in = true;
+        for (int a = 0; a < 3 && in; ++a) {
+            const double lo = e->xf0_h[3 * b + a];
+            const double hi = lo + e->n_cell * e->dx0_h[3 * b + a];
+            in = (x[a] >= lo && x[a] < hi);
+        }
+        if (!in)
+            continue;
+        ib = b;
+        for (int a = 0; a < 3; ++a) {
+            int c = (int)((x[a] - e->xf0_h[3 * b + a]) / e->dx0_h[3 * b + a]);
+            i[a] = std::min(std::max(c, 0), e->n_cell - 1);
+        }
+        return true;
+    }
+    return false;
+}
+
+void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t stream)
+{
+    const int nb = run->n_bands;
+    std::vector<unsigned char> img(nb * sizeof(kp::dband) + (nb + 1) * sizeof(uint64_t));
+    kp::dband *db = (kp::dband *)img.data();
+    uint64_t *bs = (uint64_t *)(img.data() + nb * sizeof(kp::dband));
+    bs[0] = 0;
+    for (int b = 0; b < nb; ++b) {
+        const kp_band &s = run->bands[b];
+        if (!(s.kappa_scale >= 0.0))
+            throw kp::error(KP_ERR_ARG, "kp_forward: band kappa_scale must be >= 0");
+        db[b].kappa_scale = s.kappa_scale;
+        db[b].albedo = s.albedo;
+        db[b].g = s.g;
+        db[b].flags = s.flags;
+        db[b].pad = 0;
+        bs[b + 1] = bs[b] + s.n_packets;
+    }
+    total = bs[nb];
+    if (img == e->shadow)
+        return;
+    if (nb > e->bands_cap) {
+        dfree(e->bands_d);
+        dfree(e->band_start_d);
+        if (e->stage)
+            (void)hipHostFree(e->stage);
+        e->stage = nullptr;
+        e->bands_d = dmalloc<kp::dband>(nb);
+        e->band_start_d = dmalloc<uint64_t>(nb + 1);
+        KP_HIP(hipHostMalloc(&e->stage, img.size(), hipHostMallocDefault));
+        e->bands_cap = nb;
+    }
+    // the staging buffer may still feed an earlier copy on this stream
+    KP_HIP(hipStreamSynchronize(stream));
+    std::memcpy(e->stage, img.data(), img.size());
+    KP_HIP(hipMemcpyAsync(e->bands_d, e->stage, nb * sizeof(kp::dband), hipMemcpyHostToDevice,
+                          stream));
+    KP_HIP(hipMemcpyAsync(e->band_start_d, (unsigned char *)e->stage + nb * sizeof(kp::dband),
+                          (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    KP_HIP(hipStreamSynchronize(stream));
+    e->shadow = img;
+}
+
+} // namespace
+
+extern "C" {
+
+kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **out)
+{
+    return kp::guard([&] {
+        if (!m || !out)
+            throw kp::error(KP_ERR_ARG, "kp_engine_create: null argument");
+        if (m->n_blocks < 1 || m->n_cell < 2 || (m->n_cell & 1))
+            throw kp::error(KP_ERR_ARG, "kp_engine_create: bad block table");
+        int ndev = 0;
+        KP_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev)
+            throw kp::error(KP_ERR_ARG, "kp_engine_create: no such device");
+        device_guard dg(device);
+        auto *e = new kp_engine();
+        try {
+            e->device = device;
+            e->n_blocks = m->n_blocks;
+            e->n_cell = m->n_cell;
+            e->n3 = m->n_cell * m->n_cell * m->n_cell;
+            for (int a = 0; a < 3; ++a) {
+                e->periodic[a] = m->periodic[a];
+                e->xmin[a] = m->x_min[a];
+                e->xmax[a] = m->x_max[a];
+            }
+            const int nb = m->n_blocks;
+            e->xf0_h.assign(m->xf0, m->xf0 + 3 * nb);
+            e->dx0_h.assign(m->dx0, m->dx0 + 3 * nb);
+            e->xf0 = dmalloc<double>(3 * nb);
+            e->dx0 = dmalloc<double>(3 * nb);
+            e->vol0 = dmalloc<double>(nb);
+            e->nb_id = dmalloc<int32_t>(24 * nb);
+            e->nb_shift = dmalloc<uint8_t>(24 * nb);
+            e->d_lvl = dmalloc<int8_t>(6 * nb);
+            e->phybnd = dmalloc<uint8_t>(nb);
+            e->h32 = dmalloc<float>(3 * nb);
+            e->org32 = dmalloc<float>(3 * nb);
+            e->vol32 = dmalloc<float>(nb);
+            upload(e->xf0, m->xf0, 3 * nb);
+            upload(e->dx0, m->dx0, 3 * nb);
+            upload(e->vol0, m->vol0, nb);
+            upload(e->nb_id, m->nb_id, 24 * nb);
+            upload(e->nb_shift, m->nb_shift, 24 * nb);
+            upload(e->d_lvl, m->d_lvl, 6 * nb);
+            upload(e->phybnd, m->phybnd, nb);
+            std::vector<float> h(3 * nb), o(3 * nb), v(nb);
+            for (int b = 0; b < nb; ++b) {
+                for (int a = 0; a < 3; ++a) {
+                    h[3 * b + a] = (float)m->dx0[3 * b + a];
+                    o[3 * b + a] = (float)m->xf0[3 * b + a];
+                }
+                v[b] = (float)m->vol0[b];
+            }
+            upload(e->h32, h.data(), 3 * nb);
+            upload(e->org32, o.data(), 3 * nb);
+            upload(e->vol32, v.data(), nb);
+        } catch (...) {
+            free_all(e);
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+void kp_engine_destroy(kp_engine *eng)
+{
+    if (!eng)
+        return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(eng->device);
+    (void)hipDeviceSynchronize();
+    free_all(eng);
+    (void)hipSetDevice(prev);
+    delete eng;
+}
+
+kp_status kp_set_media(kp_engine *e, const float *kappa_rho, const float *albedo, const float *g)
+{
+    return kp::guard([&] {
+        if (!e || !kappa_rho || !albedo)
+            throw kp::error(KP_ERR_ARG, "kp_set_media: null argument");
+        device_guard dg(e->device);
+        const size_t nc = (size_t)e->n_blocks * e->n3;
+        for (size_t q = 0; q < nc; ++q)
+            if (!(kappa_rho[q] >= 0.0f) || !(albedo[q] >= 0.0f && albedo[q] <= 1.0f))
+                throw kp::error(KP_ERR_ARG, "kp_set_media: kappa_rho must be >= 0 and "
+                                            "albedo in [0,1]");
+        if (!e->kappa) {
+            e->kappa = dmalloc<float>(nc);
+            e->albedo = dmalloc<float>(nc);
+            e->g = dmalloc<float>(nc);
+        }
+        upload(e->kappa, kappa_rho, nc);
+        upload(e->albedo, albedo, nc);
+        if (g) {
+            upload(e->g, g, nc);
+        } else {
+            KP_HIP(hipMemset(e->g, 0, nc * sizeof(float)));
+        }
+        e->media_set = true;
+    });
+}
+
+kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_tallies *out,
+                     void *stream_)
+{
+    return kp::guard([&] {
+        if (!e || !src || !run || !out)
+            throw kp::error(KP_ERR_ARG, "kp_forward: null argument");
+        if (!e->media_set)
+            throw kp::error(KP_ERR_STATE, "kp_forward: kp_set_media not called");
+        if (run->n_bands < 1 || !run->bands)
+            throw kp::error(KP_ERR_ARG, "kp_forward: need at least one band");
+        if (run->n_incl < 0 || run->n_incl > kp::kMaxIncl || (run->n_incl > 0 && !run->incl_deg))
+            throw kp::error(KP_ERR_ARG, "kp_forward: n_incl must be in [0,8]");
+        if (run->n_incl > 0 && (e->periodic[0] || e->periodic[1] || e->periodic[2]))
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_forward: peel-off images need a "
+                                                "non-periodic domain");
+        if (run->n_incl > 0 && out->image && (run->image_n < 1 || !(run->image_half > 0.0)))
+            throw kp::error(KP_ERR_ARG, "kp_forward: bad image geometry");
+        if (run->scatter_model != KP_THOMSON && run->scatter_model != KP_HG_RAYLEIGH)
+            throw kp::error(KP_ERR_ARG, "kp_forward: unknown scatter_model");
+        if (run->precision != KP_F64 && run->precision != KP_F32)
+            throw kp::error(KP_ERR_ARG, "kp_forward: unknown precision");
+        if (!(run->rr_survive > 0.0 && run->rr_survive <= 1.0))
+            throw kp::error(KP_ERR_ARG, "kp_forward: rr_survive must be in (0,1]");
+        device_guard dg(e->device);
+        hipStream_t stream = (hipStream_t)stream_;
+
+        kp::kparams P;
+        std::memset(&P, 0, sizeof(P));
+        P.xf0 = e->xf0;
+        P.dx0 = e->dx0;
+        P.vol0 = e->vol0;
+        P.nb_id = e->nb_id;
+        P.nb_shift = e->nb_shift;
+        P.d_lvl = e->d_lvl;
+        P.phybnd = e->phybnd;
+        P.h32 = e->h32;
+        P.org32 = e->org32;
+        P.vol32 = e->vol32;
+        P.n_cell = e->n_cell;
+        P.n3 = e->n3;
+        for (int a = 0; a < 3; ++a) {
+            P.periodic[a] = e->periodic[a];
+            P.xmin[a] = e->xmin[a];
+            P.xmax[a] = e->xmax[a];
+            P.center[a] = (e->xmin[a] + e->xmax[a]) * 0.5;
+        }
+        P.kappa = e->kappa;
+        P.albedo = e->albedo;
+        P.g = e->g;
+        uint64_t total = 0;
+        upload_bands(e, run, total, stream);
+        P.bands = e->bands_d;
+        P.band_start = e->band_start_d;
+        P.n_bands = run->n_bands;
+        P.total = total;
+        P.packet_offset = run->packet_offset;
+        int ib, ii[3];
+        if (!locate(e, src->x, ib, ii))
+            throw kp::error(KP_ERR_ARG, "kp_forward: source outside the mesh");
+        for (int a = 0; a < 3; ++a) {
+            P.src_x[a] = src->x[a];
+            P.src_i[a] = ii[a];
+            P.src_xl[a] = (float)((src->x[a] - e->xf0_h[3 * ib + a]) / e->dx0_h[3 * ib + a]);
+        }
+        P.src_ib = ib;
+        P.w0 = src->weight;
+        P.seed = run->seed;
+        P.model = run->scatter_model;
+        P.pmax = run->p_max;
+        P.step_max = run->step_max;
+        P.rr_w = run->rr_weight;
+        P.rr_s = run->rr_survive;
+        P.n_incl = run->n_incl;
+        for (int o = 0; o < run->n_incl; ++o) {
+            double s, c;
+            kp_sincos2pi(run->incl_deg[o] / 360.0, &s, &c);
+            P.no[o][0] = s;
+            P.no[o][1] = 0.0;
+            P.no[o][2] = c;
+            P.ev[o][0] = -c;
+            P.ev[o][1] = 0.0;
+            P.ev[o][2] = s;
+            P.eh[o][0] = 0.0;
+            P.eh[o][1] = 1.0;
+            P.eh[o][2] = 0.0;
+        }
+        P.image_n = run->image_n;
+        P.image_half = run->image_half;
+        P.pix_scale = run->image_n > 0 ? (double)run->image_n / (2.0 * run->image_half) : 0.0;
+        P.J = out->J;
+        P.image = (run->n_incl > 0) ? out->image : nullptr;
+        P.counters = (unsigned long long *)out->counters;
+        P.end_state = out->end_state;
+        if (total == 0)
+            return;
+        const int T = e->block_threads;
+        uint64_t blocks = (total + T - 1) / T;
+        if (e->grid_blocks > 0 && (uint64_t)e->grid_blocks < blocks)
+            blocks = e->grid_blocks;
+        if (blocks > 0x7fffffffull)
+            blocks = 0x7fffffffull;
+        if (run->precision == KP_F64)
+            hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)blocks), dim3(T), 0, stream, P);
+        else
+            hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)blocks), dim3(T), 0, stream, P);
+        KP_HIP(hipGetLastError());
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,117 @@
+"""FP64 CPU oracle of the photon-packet transport path -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py load
+this package, as the checker or the timed CPU baseline.  The product
+(kratos_polrad_amd) never imports it.
+
+The C sources restate the reference march of
+src/modules/particle/radiation/photon.h:20-125, types/proxy.h:64-121,
+types/particle_base.h:37-144 and mesh/geometry/geometry.h:68-83, plus the
+north-star physics (see kp_oracle.h for the list and the two documented
+deviations).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from kratos_polrad_amd import _abi
+
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.or_forward.restype = C.c_int
+        L.or_forward.argtypes = [C.POINTER(_abi.MeshDesc), P, P, P, C.POINTER(_abi.Source),
+                                 C.POINTER(_abi.Run), P, P, P, P, C.c_int]
+        L.or_trace.restype = C.c_int
+        L.or_trace.argtypes = [C.POINTER(_abi.MeshDesc), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.c_int, P, P, P,
+                               C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.or_locate.restype = C.c_int
+        L.or_locate.argtypes = [C.POINTER(_abi.MeshDesc), C.POINTER(C.c_double),
+                                C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.or_philox.restype = None
+        L.or_philox.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
+                                C.POINTER(C.c_uint32)]
+        for name in ("or_log", "or_exp", "or_thomson_mu"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
+        L.or_hg_mu.restype = C.c_double
+        L.or_hg_mu.argtypes = [C.c_double, C.c_double]
+        L.or_sincos2pi.restype = None
+        L.or_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def forward(mesh, kappa, albedo, g, source_x, run_spec, weight=1.0, J=True, image=True,
+            end_state=False, n_threads=None):
+    """Run the oracle on the same inputs as Engine.forward; returns numpy outputs."""
+    from kratos_polrad_amd import _abi
+    from kratos_polrad_amd.engine import source_c
+
+    n = mesh.n_cells
+    k = np.ascontiguousarray(np.asarray(kappa, dtype=np.float32).reshape(n))
+    a = np.ascontiguousarray(np.asarray(albedo, dtype=np.float32).reshape(n))
+    gg = None if g is None else np.ascontiguousarray(np.asarray(g, dtype=np.float32).reshape(n))
+    r, keep = run_spec.to_c()
+    nb = len(run_spec.bands)
+    out = {"counters": np.zeros(_abi.KP_NCOUNTERS, dtype=np.uint64)}
+    if J:
+        out["J"] = np.zeros(n * nb, dtype=np.float64)
+    if image and len(run_spec.incl_deg):
+        out["image"] = np.zeros((nb, len(run_spec.incl_deg), run_spec.image_n,
+                                 run_spec.image_n, 3), dtype=np.float64)
+    if end_state:
+        out["end_state"] = np.zeros((run_spec.total_packets, _abi.KP_ES_STRIDE))
+    src = source_c(source_x, weight)
+    if n_threads is None:
+        n_threads = os.cpu_count() or 1
+    rc = lib().or_forward(C.byref(mesh.desc), _ptr(k), _ptr(a), _ptr(gg), C.byref(src),
+                          C.byref(r), _ptr(out.get("J")), _ptr(out.get("image")),
+                          _ptr(out["counters"]), _ptr(out.get("end_state")), int(n_threads))
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"or_forward failed: {rc}")
+    return out
+
+
+def trace(mesh, x, d, max_steps=100000):
+    """Pure-streaming march of one ray: (ib[], cell[], length[], x_end, status)."""
+    ib = np.zeros(max_steps, dtype=np.int32)
+    cell = np.zeros(max_steps, dtype=np.int32)
+    ln = np.zeros(max_steps, dtype=np.float64)
+    xe = (C.c_double * 3)()
+    st = C.c_int()
+    k = lib().or_trace(C.byref(mesh.desc), (C.c_double * 3)(*x), (C.c_double * 3)(*d),
+                       max_steps, _ptr(ib), _ptr(cell), _ptr(ln), xe, C.byref(st))
+    return ib[:k], cell[:k], ln[:k], np.array(xe[:]), st.value
+
+
+def philox(ctr, k0, k1):
+    c = (C.c_uint32 * 4)(*ctr)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox(c, k0, k1, o)
+    return list(o)

@@ -1,0 +1,681 @@
+/*
+ * kp_oracle.c -- FP64 CPU oracle of the photon-packet transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see kp_oracle.h).  Built with
+ * -ffp-contract=off so every multiply and add rounds on its own; fused
+ * operations are the explicit fma() calls.  The leaf arithmetic (Philox,
+ * log, exp, sincos, scattering-angle inversion) comes from the shared
+ * deterministic header and is pinned separately against libm and the
+ * Random123 known-answer vectors.
+ */
+#define _GNU_SOURCE
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../kratos_polrad_amd/csrc/kp_math.h"
+#include "kp_oracle.h"
+
+#define TAU_CUT 40.0
+
+/* ------------------------------------------------------------------ */
+/* Leaf wrappers                                                       */
+
+void or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    kp_philox4x32_10(c, k0, k1);
+    memcpy(out, c, sizeof(c));
+}
+double or_log(double x) { return kp_log(x); }
+double or_exp(double x) { return kp_exp(x); }
+void or_sincos2pi(double u, double *s, double *c) { kp_sincos2pi(u, s, c); }
+double or_thomson_mu(double xi) { return kp_thomson_mu(xi); }
+double or_hg_mu(double g, double xi) { return kp_hg_mu(g, xi); }
+
+/* ------------------------------------------------------------------ */
+/* Ray state and the reference march                                   */
+
+typedef struct {
+    double x[3], d[3];
+    double xl[3], xr[3]; /* geo_loc_t face bounds of the current cell */
+    int i[3];
+    int ib;
+} ray_t;
+
+enum { ST_IN = 0, ST_ESCAPED = 1, ST_LOST = 4 };
+
+static double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* geo_t::x_fc for uniform axes (geometry.h:80-81) */
+static double x_fc(const kp_mesh_desc *m, int ib, int a, int i)
+{
+    return m->xf0[3 * ib + a] + (double)i * m->dx0[3 * ib + a];
+}
+
+/* geo_loc_t::load (particle_base.h:91-100) */
+static void gl_load(const kp_mesh_desc *m, ray_t *r)
+{
+    for (int a = 0; a < 3; ++a) {
+        r->xl[a] = x_fc(m, r->ib, a, r->i[a]);
+        r->xr[a] = x_fc(m, r->ib, a, r->i[a] + 1);
+    }
+}
+
+/* cart_t::get_dl (photon.h:20-44): face distance per axis, argmin with ties
+ * going to the highest axis.  Returns the axis, or -1 if no axis moves. */
+static int get_dl(const ray_t *r, double *lmin)
+{
+    double dl[3];
+    for (int a = 0; a < 3; ++a) {
+        if (r->d[a] == 0.0) {
+            dl[a] = -1.0;
+            continue;
+        }
+        const double xf = r->d[a] > 0.0 ? r->xr[a] : r->xl[a];
+        dl[a] = (xf - r->x[a]) / r->d[a];
+        if (dl[a] < 0.0)
+            dl[a] = 0.0; /* deviation (2) */
+    }
+    int ap = -1;
+    double best = DBL_MAX;
+    for (int a = 0; a < 3; ++a)
+        if (dl[a] >= 0.0 && dl[a] <= best) {
+            ap = a;
+            best = dl[a];
+        }
+    *lmin = best;
+    return ap;
+}
+
+/* proc_geo (photon.h:54-71) + shift_blk (proxy.h:64-121) + regulate
+ * (particle_base.h:37-56).  *reload is set when the cell bounds must be
+ * recomputed from the (new) block. */
+static int cross_face(const kp_mesh_desc *m, ray_t *r, int ap, double l, int *reload)
+{
+    *reload = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (a == ap) {
+            const int rhs = r->d[a] > 0.0;
+            r->x[a] = rhs ? r->xr[a] : r->xl[a];
+            r->i[a] += rhs ? 1 : -1;
+            if (rhs) {
+                r->xl[a] = r->xr[a];
+                r->xr[a] = x_fc(m, r->ib, a, r->i[a] + 1);
+            } else {
+                r->xr[a] = r->xl[a];
+                r->xl[a] = x_fc(m, r->ib, a, r->i[a]);
+            }
+        } else {
+            r->x[a] = r->x[a] + l * r->d[a];
+        }
+    }
+    const int n = m->n_cell;
+    for (int a = 0; a < 3; ++a) {
+        if (r->i[a] >= 0 && r->i[a] < n)
+            continue;
+        const int side = r->i[a] > 0;
+        if (m->phybnd[r->ib] & (1u << (2 * a + side)))
+            return ST_ESCAPED;
+        const int ap0 = (a + 1) % 3, ap1 = (a + 2) % 3;
+        const int dl = m->d_lvl[6 * r->ib + 2 * a + side];
+        int j0 = 0, j1 = 0;
+        if (dl > 0) {
+            j0 = r->i[ap0] >= n / 2;
+            j1 = r->i[ap1] >= n / 2;
+        }
+        const int slot = 24 * r->ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
+        const int nb = m->nb_id[slot];
+        if (nb < 0)
+            return ST_LOST;
+        const int shift = m->nb_shift[slot];
+        r->ib = nb;
+        r->i[a] = side ? 0 : n - 1;
+        const int apj[2] = {ap0, ap1};
+        if (dl < 0) {
+            for (int j = 0; j <= 1; ++j) {
+                const int is = r->i[apj[j]];
+                r->i[apj[j]] = is / 2 + ((shift >> j) & 1) * (n / 2);
+            }
+        } else if (dl > 0) {
+            for (int j = 0; j <= 1; ++j) {
+                const int b = apj[j];
+                const int is = r->i[b];
+                int f = (is < n / 2 ? is : is - n / 2) * 2;
+                if (r->x[b] >= x_fc(m, nb, b, f + 1)) /* deviation (1) */
+                    f += 1;
+                r->i[b] = f;
+            }
+        }
+        *reload = 1;
+        break;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (m->periodic[a]) {
+            const double lbox = m->x_max[a] - m->x_min[a];
+            if (r->x[a] < m->x_min[a]) {
+                r->x[a] += lbox;
+                *reload = 1;
+            } else if (r->x[a] >= m->x_max[a]) {
+                r->x[a] -= lbox;
+                *reload = 1;
+            }
+        }
+    return ST_IN;
+}
+
+static int cell_index(const kp_mesh_desc *m, const ray_t *r)
+{
+    const int n = m->n_cell;
+    return r->i[0] + n * (r->i[1] + n * r->i[2]);
+}
+
+int or_locate(const kp_mesh_desc *m, const double x[3], int32_t *ib, int32_t i[3])
+{
+    const int n = m->n_cell;
+    for (int b = 0; b < m->n_blocks; ++b) {
+        int in = 1;
+        for (int a = 0; a < 3 && in; ++a) {
+            const double lo = m->xf0[3 * b + a];
+            const double hi = lo + n * m->dx0[3 * b + a];
+            in = (x[a] >= lo && x[a] < hi);
+        }
+        if (!in)
+            continue;
+        *ib = b;
+        for (int a = 0; a < 3; ++a) {
+            int c = (int)((x[a] - m->xf0[3 * b + a]) / m->dx0[3 * b + a]);
+            if (c < 0) c = 0;
+            if (c > n - 1) c = n - 1;
+            i[a] = c;
+        }
+        return 0;
+    }
+    *ib = -1;
+    return -1;
+}
+
+int or_trace(const kp_mesh_desc *m, const double x[3], const double d[3], int max_steps,
+             int32_t *out_ib, int32_t *out_cell, double *out_len, double x_end[3],
+             int *status)
+{
+    ray_t r;
+    memcpy(r.x, x, sizeof(r.x));
+    memcpy(r.d, d, sizeof(r.d));
+    *status = ST_LOST;
+    if (or_locate(m, x, &r.ib, r.i) != 0)
+        return 0;
+    gl_load(m, &r);
+    int k = 0;
+    *status = 0;
+    while (k < max_steps) {
+        double l;
+        const int ap = get_dl(&r, &l);
+        if (ap < 0) {
+            *status = ST_LOST;
+            break;
+        }
+        out_ib[k] = r.ib;
+        out_cell[k] = cell_index(m, &r);
+        out_len[k] = l;
+        ++k;
+        int reload;
+        const int st = cross_face(m, &r, ap, l, &reload);
+        if (st != ST_IN) {
+            *status = st;
+            break;
+        }
+        if (reload)
+            gl_load(m, &r);
+    }
+    memcpy(x_end, r.x, sizeof(r.x));
+    return k;
+}
+
+/* ------------------------------------------------------------------ */
+/* Forward transport                                                   */
+
+typedef struct {
+    const kp_mesh_desc *m;
+    const float *kappa, *albedo, *g;
+    const kp_source *src;
+    const kp_run *run;
+    int32_t src_ib, src_i[3];
+    int n_incl;
+    double no[8][3], ev[8][3], eh[8][3];
+    double center[3];
+    double pix_scale;
+    uint64_t total;
+    uint64_t band_start[1024];
+} fctx_t;
+
+typedef struct {
+    const fctx_t *c;
+    uint64_t k_lo, k_hi; /* global packet slots */
+    double *J, *image;
+    uint64_t cnt[KP_NCOUNTERS];
+    double *end_state;
+} fthr_t;
+
+/* optical depth from r to the domain edge along d (peel-off ray) */
+static double tau_to_edge(const fctx_t *c, const ray_t *from, const double *dir, double ks,
+                          uint64_t *ncross)
+{
+    const kp_mesh_desc *m = c->m;
+    const int n3 = m->n_cell * m->n_cell * m->n_cell;
+    ray_t r = *from;
+    memcpy(r.d, dir, sizeof(r.d));
+    double tau = 0.0;
+    for (;;) {
+        double l;
+        const int ap = get_dl(&r, &l);
+        if (ap < 0)
+            break;
+        const double kap = (double)c->kappa[(size_t)r.ib * n3 + cell_index(m, &r)] * ks;
+        tau = tau + kap * l;
+        *ncross += 1;
+        if (tau > TAU_CUT)
+            break;
+        int reload;
+        const int st = cross_face(m, &r, ap, l, &reload);
+        if (st != ST_IN)
+            break;
+        if (reload)
+            gl_load(m, &r);
+    }
+    return tau;
+}
+
+static void deposit(const fctx_t *c, fthr_t *t, int band, int o, const double *x, double I,
+                    double Q, double U)
+{
+    if (!t->image)
+        return;
+    const int N = c->run->image_n;
+    double rel[3];
+    for (int a = 0; a < 3; ++a)
+        rel[a] = x[a] - c->center[a];
+    const double uh = dot3(rel, c->eh[o]);
+    const double uv = dot3(rel, c->ev[o]);
+    const double fx = floor((uh + c->run->image_half) * c->pix_scale);
+    const double fy = floor((uv + c->run->image_half) * c->pix_scale);
+    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+        return;
+    const size_t pix = (((size_t)band * c->n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
+    t->image[3 * pix + 0] += I;
+    t->image[3 * pix + 1] += Q;
+    t->image[3 * pix + 2] += U;
+}
+
+/* normalized phase-matrix elements at scattering cosine mu */
+static void phase(int model, double gg, double pmax, double mu, double *z11, double *z12,
+                  double *z33)
+{
+    const double m2 = mu * mu;
+    if (model == KP_THOMSON) {
+        *z11 = 0.75 * (1.0 + m2);
+        *z12 = 0.75 * (m2 - 1.0);
+        *z33 = 1.5 * mu;
+    } else {
+        const double D = (1.0 + gg * gg) - 2.0 * gg * mu;
+        const double ph = (1.0 - gg * gg) / (D * sqrt(D));
+        const double pol = pmax * (1.0 - m2) / (1.0 + m2);
+        *z11 = ph;
+        *z12 = -pol * ph;
+        *z33 = ph * (2.0 * mu / (1.0 + m2));
+    }
+}
+
+static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t slot)
+{
+    const kp_mesh_desc *m = c->m;
+    const kp_run *run = c->run;
+    const kp_band *bd = &run->bands[band];
+    const int n3 = m->n_cell * m->n_cell * m->n_cell;
+    const int nbands = run->n_bands;
+    const double ks = bd->kappa_scale;
+    const double w0 = c->src->weight;
+    const uint32_t k0 = run->seed, k1 = (uint32_t)band;
+    const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
+
+    uint64_t ncross = 0, npeel = 0;
+    int events = 0, status = 0;
+
+    /* emission (event 0) */
+    uint32_t rn[4] = {plo, phi, 0u, 0u};
+    kp_philox4x32_10(rn, k0, k1);
+    ray_t r;
+    double e1[3];
+    {
+        const double mu = fma(2.0, kp_u01(rn[0]), -1.0);
+        double sf, cf;
+        kp_sincos2pi(kp_u01(rn[1]), &sf, &cf);
+        double s2 = fma(-mu, mu, 1.0);
+        if (s2 < 0.0) s2 = 0.0;
+        const double st = sqrt(s2);
+        r.d[0] = st * cf;
+        r.d[1] = st * sf;
+        r.d[2] = mu;
+        e1[0] = mu * cf;
+        e1[1] = mu * sf;
+        e1[2] = -st;
+    }
+    double tau_rem = -kp_log(kp_u01(rn[2]));
+    double I = w0, Q = 0.0, U = 0.0;
+    memcpy(r.x, c->src->x, sizeof(r.x));
+    r.ib = c->src_ib;
+    memcpy(r.i, c->src_i, sizeof(r.i));
+    gl_load(m, &r);
+    for (int o = 0; o < c->n_incl; ++o) {
+        const double tau = tau_to_edge(c, &r, c->no[o], ks, &npeel);
+        const double att = kp_exp(-tau) * KP_INV_4PI;
+        deposit(c, t, band, o, r.x, I * att, 0.0, 0.0);
+    }
+
+    int steps = run->step_max;
+    int cell = 0;
+    for (;;) {
+        /* march to the next interaction or out of the domain */
+        int interact = 0;
+        for (;;) {
+            if (steps <= 0) {
+                status = KP_END_STEPS;
+                break;
+            }
+            double l;
+            const int ap = get_dl(&r, &l);
+            if (ap < 0) {
+                status = KP_END_LOST;
+                break;
+            }
+            cell = cell_index(m, &r);
+            const size_t gc = (size_t)r.ib * n3 + cell;
+            const double kap = (double)c->kappa[gc] * ks;
+            const double dtau = kap * l;
+            const double V = m->vol0[r.ib];
+            ncross += 1;
+            steps -= 1;
+            if (kap > 0.0 && dtau >= tau_rem) {
+                double li = tau_rem / kap;
+                if (li > l)
+                    li = l;
+                if (t->J)
+                    t->J[gc * nbands + band] += I * (li / V);
+                for (int a = 0; a < 3; ++a)
+                    r.x[a] = r.x[a] + li * r.d[a];
+                interact = 1;
+                break;
+            }
+            tau_rem = tau_rem - dtau;
+            if (t->J)
+                t->J[gc * nbands + band] += I * (l / V);
+            int reload;
+            const int st = cross_face(m, &r, ap, l, &reload);
+            if (st == ST_ESCAPED) {
+                status = KP_END_ESCAPED;
+                break;
+            }
+            if (st == ST_LOST) {
+                status = KP_END_LOST;
+                break;
+            }
+            if (reload)
+                gl_load(m, &r);
+        }
+        if (!interact)
+            break;
+
+        /* interaction */
+        events += 1;
+        uint32_t u[4] = {plo, phi, (uint32_t)events, 0u};
+        kp_philox4x32_10(u, k0, k1);
+        const size_t gc = (size_t)r.ib * n3 + cell;
+        const double alb = (bd->flags & KP_BAND_CELL_ALBEDO) ? (double)c->albedo[gc] : bd->albedo;
+        const double gg = (bd->flags & KP_BAND_CELL_G) ? (c->g ? (double)c->g[gc] : 0.0) : bd->g;
+        I = I * alb;
+        Q = Q * alb;
+        U = U * alb;
+
+        double e2[3];
+        cross3(r.d, e1, e2);
+
+        /* peel-off toward every observer */
+        for (int o = 0; o < c->n_incl; ++o) {
+            const double *no = c->no[o];
+            double mo = dot3(r.d, no);
+            if (mo > 1.0) mo = 1.0;
+            if (mo < -1.0) mo = -1.0;
+            double p[3];
+            for (int a = 0; a < 3; ++a)
+                p[a] = no[a] - mo * r.d[a];
+            const double pn2 = dot3(p, p);
+            double cf = 1.0, sf = 0.0, st = 0.0;
+            if (pn2 > 1e-24) {
+                const double ip = 1.0 / sqrt(pn2);
+                cf = dot3(p, e1) * ip;
+                sf = dot3(p, e2) * ip;
+                st = pn2 * ip;
+            }
+            const double c2 = cf * cf - sf * sf;
+            const double s2 = 2.0 * cf * sf;
+            const double Qr = Q * c2 + U * s2;
+            const double Ur = U * c2 - Q * s2;
+            double z11, z12, z33;
+            phase(run->scatter_model, gg, run->p_max, mo, &z11, &z12, &z33);
+            const double Ip = z11 * I + z12 * Qr;
+            const double Qp = z12 * I + z11 * Qr;
+            const double Up = z33 * Ur;
+            double e1o[3], e2o[3];
+            for (int a = 0; a < 3; ++a) {
+                const double e1r = cf * e1[a] + sf * e2[a];
+                e1o[a] = mo * e1r - st * r.d[a];
+            }
+            cross3(no, e1o, e2o);
+            const double cps = dot3(c->ev[o], e1o);
+            const double sps = dot3(c->ev[o], e2o);
+            const double nrm = cps * cps + sps * sps;
+            const double c2p = (cps * cps - sps * sps) / nrm;
+            const double s2p = 2.0 * cps * sps / nrm;
+            const double Qi = Qp * c2p + Up * s2p;
+            const double Ui = Up * c2p - Qp * s2p;
+            const double tau = tau_to_edge(c, &r, no, ks, &npeel);
+            const double att = kp_exp(-tau) * KP_INV_4PI;
+            deposit(c, t, band, o, r.x, Ip * att, Qi * att, Ui * att);
+        }
+
+        /* scatter: sample from the unpolarized phase function, carry the
+         * polarization in the Stokes weight */
+        {
+            const double mu = (run->scatter_model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
+                                                                  : kp_hg_mu(gg, kp_u01(u[0]));
+            double s2m = fma(-mu, mu, 1.0);
+            if (s2m < 0.0) s2m = 0.0;
+            const double st = sqrt(s2m);
+            double sf, cf;
+            kp_sincos2pi(kp_u01(u[1]), &sf, &cf);
+            const double c2 = cf * cf - sf * sf;
+            const double s2 = 2.0 * cf * sf;
+            const double Qr = Q * c2 + U * s2;
+            const double Ur = U * c2 - Q * s2;
+            const double m2 = mu * mu;
+            const double pl = (run->scatter_model == KP_THOMSON) ? 1.0 : run->p_max;
+            const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
+            const double a33 = 2.0 * mu / (1.0 + m2);
+            const double In = I + a12 * Qr;
+            const double Qn = a12 * I + Qr;
+            const double Un = a33 * Ur;
+            I = In;
+            Q = Qn;
+            U = Un;
+            double dn[3], en[3];
+            for (int a = 0; a < 3; ++a) {
+                const double e1r = cf * e1[a] + sf * e2[a];
+                dn[a] = mu * r.d[a] + st * e1r;
+                en[a] = mu * e1r - st * r.d[a];
+            }
+            const double id = 1.0 / sqrt(dot3(dn, dn));
+            for (int a = 0; a < 3; ++a)
+                dn[a] = dn[a] * id;
+            const double pr = dot3(en, dn);
+            for (int a = 0; a < 3; ++a)
+                en[a] = en[a] - pr * dn[a];
+            const double ie = 1.0 / sqrt(dot3(en, en));
+            for (int a = 0; a < 3; ++a) {
+                r.d[a] = dn[a];
+                e1[a] = en[a] * ie;
+            }
+        }
+        tau_rem = -kp_log(kp_u01(u[2]));
+        if (I < run->rr_weight * w0) {
+            if (kp_u01(u[3]) < run->rr_survive) {
+                I = I / run->rr_survive;
+                Q = Q / run->rr_survive;
+                U = U / run->rr_survive;
+            } else {
+                status = KP_END_RR;
+                break;
+            }
+        }
+    }
+
+    t->cnt[KP_C_CROSS_MAIN] += ncross;
+    t->cnt[KP_C_CROSS_PEEL] += npeel;
+    t->cnt[KP_C_EVENTS] += (uint64_t)events;
+    t->cnt[KP_C_PACKETS] += 1;
+    if (status == KP_END_ESCAPED) t->cnt[KP_C_ESCAPED] += 1;
+    else if (status == KP_END_RR) t->cnt[KP_C_KILLED_RR] += 1;
+    else if (status == KP_END_STEPS) t->cnt[KP_C_KILLED_STEPS] += 1;
+    else t->cnt[KP_C_LOST] += 1;
+    if (t->end_state) {
+        double *es = t->end_state + slot * KP_ES_STRIDE;
+        for (int a = 0; a < 3; ++a) {
+            es[KP_ES_X + a] = r.x[a];
+            es[KP_ES_D + a] = r.d[a];
+        }
+        es[KP_ES_I] = I;
+        es[KP_ES_Q] = Q;
+        es[KP_ES_U] = U;
+        es[KP_ES_EVENTS] = (double)events;
+        es[KP_ES_CROSS] = (double)ncross;
+        es[KP_ES_STATUS] = (double)status;
+        es[KP_ES_IB] = (double)r.ib;
+        es[KP_ES_CELL] = (double)cell_index(m, &r);
+        es[14] = (double)npeel;
+        es[15] = 0.0;
+    }
+}
+
+static void *worker(void *arg)
+{
+    fthr_t *t = (fthr_t *)arg;
+    const fctx_t *c = t->c;
+    int band = 0;
+    for (uint64_t k = t->k_lo; k < t->k_hi; ++k) {
+        while (k >= c->band_start[band + 1])
+            ++band;
+        const uint64_t pid = c->run->packet_offset + (k - c->band_start[band]);
+        packet(c, t, band, pid, k);
+    }
+    return NULL;
+}
+
+int or_forward(const kp_mesh_desc *m, const float *kappa, const float *albedo, const float *g,
+               const kp_source *src, const kp_run *run, double *J, double *image,
+               uint64_t *counters, double *end_state, int n_threads)
+{
+    if (!m || !kappa || !src || !run || run->n_bands < 1 || run->n_bands > 1023)
+        return -1;
+    if (run->n_incl < 0 || run->n_incl > 8)
+        return -1;
+    fctx_t *c = (fctx_t *)calloc(1, sizeof(fctx_t));
+    c->m = m;
+    c->kappa = kappa;
+    c->albedo = albedo;
+    c->g = g;
+    c->src = src;
+    c->run = run;
+    if (or_locate(m, src->x, &c->src_ib, c->src_i) != 0) {
+        free(c);
+        return -2;
+    }
+    c->n_incl = run->n_incl;
+    for (int o = 0; o < run->n_incl; ++o) {
+        double s, co;
+        kp_sincos2pi(run->incl_deg[o] / 360.0, &s, &co);
+        c->no[o][0] = s;  c->no[o][1] = 0.0; c->no[o][2] = co;
+        c->ev[o][0] = -co; c->ev[o][1] = 0.0; c->ev[o][2] = s;
+        c->eh[o][0] = 0.0; c->eh[o][1] = 1.0; c->eh[o][2] = 0.0;
+    }
+    for (int a = 0; a < 3; ++a)
+        c->center[a] = (m->x_min[a] + m->x_max[a]) * 0.5;
+    c->pix_scale = run->image_n > 0 ? (double)run->image_n / (2.0 * run->image_half) : 0.0;
+    c->band_start[0] = 0;
+    for (int b = 0; b < run->n_bands; ++b)
+        c->band_start[b + 1] = c->band_start[b] + run->bands[b].n_packets;
+    c->total = c->band_start[run->n_bands];
+    for (int b = run->n_bands + 1; b < 1024; ++b)
+        c->band_start[b] = c->total;
+
+    if (n_threads < 1)
+        n_threads = 1;
+    if ((uint64_t)n_threads > c->total && c->total > 0)
+        n_threads = (int)c->total;
+    const size_t ncell = (size_t)m->n_blocks * m->n_cell * m->n_cell * m->n_cell;
+    const size_t nJ = ncell * run->n_bands;
+    const size_t nimg = (size_t)run->n_bands * run->n_incl * run->image_n * run->image_n * 3;
+    fthr_t *th = (fthr_t *)calloc((size_t)n_threads, sizeof(fthr_t));
+    pthread_t *tid = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+    int rc = 0;
+    for (int k = 0; k < n_threads; ++k) {
+        th[k].c = c;
+        th[k].k_lo = c->total * k / n_threads;
+        th[k].k_hi = c->total * (k + 1) / n_threads;
+        th[k].end_state = end_state;
+        if (J) {
+            th[k].J = (k == 0) ? J : (double *)calloc(nJ, sizeof(double));
+            if (!th[k].J) rc = -3;
+        }
+        if (image && nimg) {
+            th[k].image = (k == 0) ? image : (double *)calloc(nimg, sizeof(double));
+            if (!th[k].image) rc = -3;
+        }
+    }
+    if (rc == 0) {
+        for (int k = 1; k < n_threads; ++k)
+            pthread_create(&tid[k], NULL, worker, &th[k]);
+        worker(&th[0]);
+        for (int k = 1; k < n_threads; ++k)
+            pthread_join(tid[k], NULL);
+        for (int k = 1; k < n_threads; ++k) {
+            if (J)
+                for (size_t q = 0; q < nJ; ++q)
+                    J[q] += th[k].J[q];
+            if (image && nimg)
+                for (size_t q = 0; q < nimg; ++q)
+                    image[q] += th[k].image[q];
+        }
+        if (counters)
+            for (int k = 0; k < n_threads; ++k)
+                for (int q = 0; q < KP_NCOUNTERS; ++q)
+                    counters[q] += th[k].cnt[q];
+    }
+    for (int k = 1; k < n_threads; ++k) {
+        if (J) free(th[k].J);
+        if (image && nimg) free(th[k].image);
+    }
+    free(th);
+    free(tid);
+    free(c);
+    return rc;
+}

@@ -1,0 +1,68 @@
+/*
+ * kp_oracle.h -- FP64 CPU oracle of the photon-packet transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or the CPU baseline; the product path never calls it.
+ *
+ * It restates, in plain C with pthreads over packet-id ranges:
+ *  - the reference march: get_dl (photon.h:20-44), proc_phys tally
+ *    (photon.h:73-83), proc_geo snap (photon.h:54-71), the step budget and
+ *    block reload of proc (photon.h:104-125), shift_blk level jumps
+ *    (proxy.h:64-121), x_fc (geometry.h:68-83), periodic regulate
+ *    (particle_base.h:37-56);
+ *  - the north-star physics with no reference code, from published methods:
+ *    Philox4x32-10 keyed by (seed, band) with counter (packet id, event),
+ *    tau = -ln xi free flights, survival-biased absorption with Russian
+ *    roulette, Thomson and Henyey-Greenstein/Rayleigh-form (White 1979)
+ *    Mueller scattering with Stokes frame rotation (Chandrasekhar 1960),
+ *    peel-off images (Yusef-Zadeh, Morris & White 1984) and the path-length
+ *    mean-intensity tally (Lucy 1999).
+ * Deviations from the reference march, both inert on the dyadic meshes the
+ * reference produces for pure streaming:
+ *  (1) after a coarse->fine face crossing the tangential fine cell is the one
+ *      containing the position (the reference always takes the lower one,
+ *      proxy.h:383-389, and geo_loc_t::fix, particle_base.h:119-127, does
+ *      not update the index);
+ *  (2) a negative face distance on a moving axis (position one ulp past a
+ *      face after an interaction) is clamped to zero.
+ */
+#ifndef KP_ORACLE_H
+#define KP_ORACLE_H
+
+#include "../include/kpolrad.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Full forward run.  Outputs are zeroed by the caller and accumulated.
+ * Any of J, image, counters, end_state may be NULL. */
+int or_forward(const kp_mesh_desc *mesh, const float *kappa, const float *albedo,
+               const float *g, const kp_source *src, const kp_run *run, double *J,
+               double *image, uint64_t *counters, double *end_state, int n_threads);
+
+/* Pure-streaming trace of one ray (the reference photon.h:104-125 march with
+ * no medium): records every step's block, in-block cell index and length.
+ * Returns the number of steps written (<= max_steps); *status is 1 when
+ * the ray left through a physical boundary, 4 when lost, 0 when truncated. */
+int or_trace(const kp_mesh_desc *mesh, const double x[3], const double d[3],
+             int max_steps, int32_t *out_ib, int32_t *out_cell, double *out_len,
+             double x_end[3], int *status);
+
+/* Source cell by brute-force point location (geo_t::idx, geometry.h:120-140). */
+int or_locate(const kp_mesh_desc *mesh, const double x[3], int32_t *ib, int32_t i[3]);
+
+/* Leaf arithmetic exposed for pinning against libm / published vectors. */
+void or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
+double or_log(double x);
+double or_exp(double x);
+void or_sincos2pi(double u, double *s, double *c);
+double or_thomson_mu(double xi);
+double or_hg_mu(double g, double xi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
