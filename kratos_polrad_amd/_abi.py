@@ -127,6 +127,14 @@ def lib():
     """The loaded engine library (raises if it was never built)."""
     global _lib
     if _lib is None:
+        # torch bundles its own HIP runtime under the same SONAME
+        # (libamdhip64.so.7); importing it first makes the dynamic loader bind
+        # this library to that copy, so the process holds one HIP runtime and
+        # torch-allocated device pointers are valid in our kernels.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m kratos_polrad_amd.build`")

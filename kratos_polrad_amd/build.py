@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libkpolrad.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", os.path.join(ROOT, "include")]
-DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE = [f"--offload-arch={ARCH}", "-mcode-object-version=5", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = [
     ("kp_engine.hip", True),
@@ -53,7 +53,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda s: _compile(*s), SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-mcode-object-version=5", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -163,14 +163,14 @@ KP_HD void kp_sincos2pi(double u, double *s_out, double *c_out)
     sp = fma(sp, z, 1.0 / 120.0);            /* +1/5!  */
     sp = fma(sp, z, -1.0 / 6.0);             /* -1/3!  */
     const double sn = fma(x * z, sp, x);
-    double cp = 1.0 / 6402373705728000.0;    /* +1/18! */
-    cp = fma(cp, z, -1.0 / 20922789888000.0); /* -1/16! */
-    cp = fma(cp, z, 1.0 / 87178291200.0);     /* +1/14! */
-    cp = fma(cp, z, -1.0 / 479001600.0);      /* -1/12! */
-    cp = fma(cp, z, 1.0 / 3628800.0);         /* +1/10! */
-    cp = fma(cp, z, -1.0 / 40320.0);          /* -1/8!  */
-    cp = fma(cp, z, 1.0 / 720.0);             /* +1/6!  */
-    cp = fma(cp, z, -1.0 / 24.0);             /* -1/4!  */
+    double cp = -1.0 / 6402373705728000.0;   /* -1/18! */
+    cp = fma(cp, z, 1.0 / 20922789888000.0);  /* +1/16! */
+    cp = fma(cp, z, -1.0 / 87178291200.0);    /* -1/14! */
+    cp = fma(cp, z, 1.0 / 479001600.0);       /* +1/12! */
+    cp = fma(cp, z, -1.0 / 3628800.0);        /* -1/10! */
+    cp = fma(cp, z, 1.0 / 40320.0);           /* +1/8!  */
+    cp = fma(cp, z, -1.0 / 720.0);            /* -1/6!  */
+    cp = fma(cp, z, 1.0 / 24.0);              /* +1/4!  */
     cp = fma(cp, z, -0.5);                    /* -1/2!  */
     const double cs = fma(z, cp, 1.0);
     const int qi = ((int)q) & 3;

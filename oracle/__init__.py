@@ -58,6 +58,11 @@ def lib():
         L.or_hg_mu.argtypes = [C.c_double, C.c_double]
         L.or_sincos2pi.restype = None
         L.or_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        D3 = C.POINTER(C.c_double)
+        L.or_scatter_stokes.restype = None
+        L.or_scatter_stokes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D3, D3, D3]
+        L.or_peel_stokes.restype = None
+        L.or_peel_stokes.argtypes = [C.c_int, C.c_double, C.c_double, D3, D3, D3, D3, D3, D3]
         _lib = L
     return _lib
 
@@ -115,3 +120,21 @@ def philox(ctr, k0, k1):
     o = (C.c_uint32 * 4)()
     lib().or_philox(c, k0, k1, o)
     return list(o)
+
+
+def scatter_stokes(model, pmax, mu, phi_u, d, e1, S):
+    """One scattering's Stokes/frame algebra: returns (d', e1', S')."""
+    dd = (C.c_double * 3)(*d)
+    ee = (C.c_double * 3)(*e1)
+    ss = (C.c_double * 3)(*S)
+    lib().or_scatter_stokes(int(model), float(pmax), float(mu), float(phi_u), dd, ee, ss)
+    return np.array(dd[:]), np.array(ee[:]), np.array(ss[:])
+
+
+def peel_stokes(model, g, pmax, d, e1, S, no, ev):
+    """Stokes vector peeled toward no, in the image frame (Q>0 along ev)."""
+    out = (C.c_double * 3)()
+    lib().or_peel_stokes(int(model), float(g), float(pmax), (C.c_double * 3)(*d),
+                         (C.c_double * 3)(*e1), (C.c_double * 3)(*S), (C.c_double * 3)(*no),
+                         (C.c_double * 3)(*ev), out)
+    return np.array(out[:])

@@ -337,6 +337,115 @@ static void phase(int model, double gg, double pmax, double mu, double *z11, dou
     }
 }
 
+/* Stokes vector (I,Q,U) referenced to e1 (Q>0: polarized along e1), e2 = d x e1.
+ * Rotation of the reference axis by phi: Q' = Q cos2phi + U sin2phi,
+ * U' = U cos2phi - Q sin2phi (Chandrasekhar 1960, Radiative Transfer, ch. I). */
+
+/* Peel-off toward observer direction no with image vertical axis ev: the
+ * scattered Stokes vector Z(mu_o) R(phi_o) S in the image frame (Q>0 along
+ * ev), without the 1/(4 pi) and attenuation factors. */
+static void peel_stokes(int model, double gg, double pmax, const double *d, const double *e1,
+                        const double *e2, const double *S, const double *no, const double *ev,
+                        double *Si)
+{
+    double mo = dot3(d, no);
+    if (mo > 1.0) mo = 1.0;
+    if (mo < -1.0) mo = -1.0;
+    double p[3];
+    for (int a = 0; a < 3; ++a)
+        p[a] = no[a] - mo * d[a];
+    const double pn2 = dot3(p, p);
+    double cf = 1.0, sf = 0.0, st = 0.0;
+    if (pn2 > 1e-24) {
+        const double ip = 1.0 / sqrt(pn2);
+        cf = dot3(p, e1) * ip;
+        sf = dot3(p, e2) * ip;
+        st = pn2 * ip;
+    }
+    const double c2 = cf * cf - sf * sf;
+    const double s2 = 2.0 * cf * sf;
+    const double Qr = S[1] * c2 + S[2] * s2;
+    const double Ur = S[2] * c2 - S[1] * s2;
+    double z11, z12, z33;
+    phase(model, gg, pmax, mo, &z11, &z12, &z33);
+    const double Ip = z11 * S[0] + z12 * Qr;
+    const double Qp = z12 * S[0] + z11 * Qr;
+    const double Up = z33 * Ur;
+    double e1o[3], e2o[3];
+    for (int a = 0; a < 3; ++a) {
+        const double e1r = cf * e1[a] + sf * e2[a];
+        e1o[a] = mo * e1r - st * d[a];
+    }
+    cross3(no, e1o, e2o);
+    const double cps = dot3(ev, e1o);
+    const double sps = dot3(ev, e2o);
+    const double nrm = cps * cps + sps * sps;
+    const double c2p = (cps * cps - sps * sps) / nrm;
+    const double s2p = 2.0 * cps * sps / nrm;
+    Si[0] = Ip;
+    Si[1] = Qp * c2p + Up * s2p;
+    Si[2] = Up * c2p - Qp * s2p;
+}
+
+/* Scattering by cos(theta)=mu and azimuth 2 pi phi_u measured from e1
+ * toward e2: new direction and frame in d/e1, Stokes weight Z S / Z11. */
+static void scatter_stokes(int model, double pmax, double mu, double phi_u, double *d,
+                           double *e1, const double *e2, double *S)
+{
+    double s2m = fma(-mu, mu, 1.0);
+    if (s2m < 0.0) s2m = 0.0;
+    const double st = sqrt(s2m);
+    double sf, cf;
+    kp_sincos2pi(phi_u, &sf, &cf);
+    const double c2 = cf * cf - sf * sf;
+    const double s2 = 2.0 * cf * sf;
+    const double Qr = S[1] * c2 + S[2] * s2;
+    const double Ur = S[2] * c2 - S[1] * s2;
+    const double m2 = mu * mu;
+    const double pl = (model == KP_THOMSON) ? 1.0 : pmax;
+    const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
+    const double a33 = 2.0 * mu / (1.0 + m2);
+    const double In = S[0] + a12 * Qr;
+    const double Qn = a12 * S[0] + Qr;
+    const double Un = a33 * Ur;
+    S[0] = In;
+    S[1] = Qn;
+    S[2] = Un;
+    double dn[3], en[3];
+    for (int a = 0; a < 3; ++a) {
+        const double e1r = cf * e1[a] + sf * e2[a];
+        dn[a] = mu * d[a] + st * e1r;
+        en[a] = mu * e1r - st * d[a];
+    }
+    const double id = 1.0 / sqrt(dot3(dn, dn));
+    for (int a = 0; a < 3; ++a)
+        dn[a] = dn[a] * id;
+    const double pr = dot3(en, dn);
+    for (int a = 0; a < 3; ++a)
+        en[a] = en[a] - pr * dn[a];
+    const double ie = 1.0 / sqrt(dot3(en, en));
+    for (int a = 0; a < 3; ++a) {
+        d[a] = dn[a];
+        e1[a] = en[a] * ie;
+    }
+}
+
+void or_peel_stokes(int model, double g, double pmax, const double d[3], const double e1[3],
+                    const double S[3], const double no[3], const double ev[3], double Si[3])
+{
+    double e2[3];
+    cross3(d, e1, e2);
+    peel_stokes(model, g, pmax, d, e1, e2, S, no, ev, Si);
+}
+
+void or_scatter_stokes(int model, double pmax, double mu, double phi_u, double d[3],
+                       double e1[3], double S[3])
+{
+    double e2[3];
+    cross3(d, e1, e2);
+    scatter_stokes(model, pmax, mu, phi_u, d, e1, e2, S);
+}
+
 static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t slot)
 {
     const kp_mesh_desc *m = c->m;
@@ -452,46 +561,12 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
 
         /* peel-off toward every observer */
         for (int o = 0; o < c->n_incl; ++o) {
-            const double *no = c->no[o];
-            double mo = dot3(r.d, no);
-            if (mo > 1.0) mo = 1.0;
-            if (mo < -1.0) mo = -1.0;
-            double p[3];
-            for (int a = 0; a < 3; ++a)
-                p[a] = no[a] - mo * r.d[a];
-            const double pn2 = dot3(p, p);
-            double cf = 1.0, sf = 0.0, st = 0.0;
-            if (pn2 > 1e-24) {
-                const double ip = 1.0 / sqrt(pn2);
-                cf = dot3(p, e1) * ip;
-                sf = dot3(p, e2) * ip;
-                st = pn2 * ip;
-            }
-            const double c2 = cf * cf - sf * sf;
-            const double s2 = 2.0 * cf * sf;
-            const double Qr = Q * c2 + U * s2;
-            const double Ur = U * c2 - Q * s2;
-            double z11, z12, z33;
-            phase(run->scatter_model, gg, run->p_max, mo, &z11, &z12, &z33);
-            const double Ip = z11 * I + z12 * Qr;
-            const double Qp = z12 * I + z11 * Qr;
-            const double Up = z33 * Ur;
-            double e1o[3], e2o[3];
-            for (int a = 0; a < 3; ++a) {
-                const double e1r = cf * e1[a] + sf * e2[a];
-                e1o[a] = mo * e1r - st * r.d[a];
-            }
-            cross3(no, e1o, e2o);
-            const double cps = dot3(c->ev[o], e1o);
-            const double sps = dot3(c->ev[o], e2o);
-            const double nrm = cps * cps + sps * sps;
-            const double c2p = (cps * cps - sps * sps) / nrm;
-            const double s2p = 2.0 * cps * sps / nrm;
-            const double Qi = Qp * c2p + Up * s2p;
-            const double Ui = Up * c2p - Qp * s2p;
-            const double tau = tau_to_edge(c, &r, no, ks, &npeel);
+            double S[3] = {I, Q, U}, Si[3];
+            peel_stokes(run->scatter_model, gg, run->p_max, r.d, e1, e2, S, c->no[o], c->ev[o],
+                        Si);
+            const double tau = tau_to_edge(c, &r, c->no[o], ks, &npeel);
             const double att = kp_exp(-tau) * KP_INV_4PI;
-            deposit(c, t, band, o, r.x, Ip * att, Qi * att, Ui * att);
+            deposit(c, t, band, o, r.x, Si[0] * att, Si[1] * att, Si[2] * att);
         }
 
         /* scatter: sample from the unpolarized phase function, carry the
@@ -499,42 +574,11 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
         {
             const double mu = (run->scatter_model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
                                                                   : kp_hg_mu(gg, kp_u01(u[0]));
-            double s2m = fma(-mu, mu, 1.0);
-            if (s2m < 0.0) s2m = 0.0;
-            const double st = sqrt(s2m);
-            double sf, cf;
-            kp_sincos2pi(kp_u01(u[1]), &sf, &cf);
-            const double c2 = cf * cf - sf * sf;
-            const double s2 = 2.0 * cf * sf;
-            const double Qr = Q * c2 + U * s2;
-            const double Ur = U * c2 - Q * s2;
-            const double m2 = mu * mu;
-            const double pl = (run->scatter_model == KP_THOMSON) ? 1.0 : run->p_max;
-            const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
-            const double a33 = 2.0 * mu / (1.0 + m2);
-            const double In = I + a12 * Qr;
-            const double Qn = a12 * I + Qr;
-            const double Un = a33 * Ur;
-            I = In;
-            Q = Qn;
-            U = Un;
-            double dn[3], en[3];
-            for (int a = 0; a < 3; ++a) {
-                const double e1r = cf * e1[a] + sf * e2[a];
-                dn[a] = mu * r.d[a] + st * e1r;
-                en[a] = mu * e1r - st * r.d[a];
-            }
-            const double id = 1.0 / sqrt(dot3(dn, dn));
-            for (int a = 0; a < 3; ++a)
-                dn[a] = dn[a] * id;
-            const double pr = dot3(en, dn);
-            for (int a = 0; a < 3; ++a)
-                en[a] = en[a] - pr * dn[a];
-            const double ie = 1.0 / sqrt(dot3(en, en));
-            for (int a = 0; a < 3; ++a) {
-                r.d[a] = dn[a];
-                e1[a] = en[a] * ie;
-            }
+            double S[3] = {I, Q, U};
+            scatter_stokes(run->scatter_model, run->p_max, mu, kp_u01(u[1]), r.d, e1, e2, S);
+            I = S[0];
+            Q = S[1];
+            U = S[2];
         }
         tau_rem = -kp_log(kp_u01(u[2]));
         if (I < run->rr_weight * w0) {

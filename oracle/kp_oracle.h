@@ -53,6 +53,14 @@ int or_trace(const kp_mesh_desc *mesh, const double x[3], const double d[3],
 /* Source cell by brute-force point location (geo_t::idx, geometry.h:120-140). */
 int or_locate(const kp_mesh_desc *mesh, const double x[3], int32_t *ib, int32_t i[3]);
 
+/* Stokes algebra of one scattering (new d, e1, S in place; azimuth 2 pi phi_u
+ * from e1 toward e2 = d x e1) and of one peel-off toward no with image
+ * vertical axis ev (Si in the image frame, no 1/4pi, no attenuation). */
+void or_scatter_stokes(int model, double pmax, double mu, double phi_u, double d[3],
+                       double e1[3], double S[3]);
+void or_peel_stokes(int model, double g, double pmax, const double d[3], const double e1[3],
+                    const double S[3], const double no[3], const double ev[3], double Si[3]);
+
 /* Leaf arithmetic exposed for pinning against libm / published vectors. */
 void or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
 double or_log(double x);
