@@ -194,6 +194,24 @@ typedef struct kp_tallies {
     double *end_state;  /* [sum of n_packets][KP_ES_STRIDE], band-major    */
 } kp_tallies;
 
+/* Launch schedule of kp_forward.
+ * KP_SCHED_HISTORY: one thread per packet, the integrator's launch shape
+ *   (ceil(n/threads) blocks, integrator.h:40-46), the whole history in that
+ *   thread (photon.h:104-125).
+ * KP_SCHED_PERSISTENT: CUs*blocks_per_cu resident blocks; each lane runs a
+ *   one-crossing-per-iteration state machine and refills from a wave slot
+ *   range of `chunk` packets claimed with one atomic.  Same histories. */
+enum { KP_SCHED_HISTORY = 0, KP_SCHED_PERSISTENT = 1 };
+typedef struct kp_schedule {
+    int32_t kind;
+    int32_t block_threads;  /* 64, 128 or 256 (the reference's [particle] n_th) */
+    int32_t blocks_per_cu;
+    int32_t chunk;
+} kp_schedule;
+
+kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);
+kp_status kp_get_schedule(const kp_engine *eng, kp_schedule *sched);
+
 /* Emit, march, scatter and peel off every packet of every band in one
  * launch on `stream` (asynchronous). */
 kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,

@@ -82,6 +82,18 @@ class Run(C.Structure):
     ]
 
 
+KP_SCHED_HISTORY, KP_SCHED_PERSISTENT = 0, 1
+
+
+class Schedule(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("block_threads", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("chunk", C.c_int32),
+    ]
+
+
 class Tallies(C.Structure):
     _fields_ = [
         ("J", C.c_void_p),
@@ -109,6 +121,8 @@ SIGNATURES = {
     "kp_engine_destroy": (None, [_P]),
     "kp_set_media": (C.c_int, [_P, _P, _P, _P]),
     "kp_forward": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.POINTER(Tallies), _P]),
+    "kp_set_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
+    "kp_get_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
 }
 
 

@@ -66,21 +66,37 @@ def gaussian_random_field(n: int, seed: int, slope: float = -11.0 / 3.0) -> np.n
 
 
 def _sample_field(field: np.ndarray, x: np.ndarray) -> np.ndarray:
-    """Nearest-cell lookup of a unit-cube periodic field at points x [m,3]."""
+    """Periodic trilinear interpolation of a unit-cube field at points x [m,3]
+    (field values sit at cell centres (i+0.5)/n)."""
     n = field.shape[0]
-    idx = np.clip((x * n).astype(np.int64), 0, n - 1)
-    return field[idx[:, 0], idx[:, 1], idx[:, 2]]
+    u = x * n - 0.5
+    i0 = np.floor(u).astype(np.int64)
+    f = u - i0
+    out = np.zeros(len(x))
+    for dx in (0, 1):
+        wx = f[:, 0] if dx else 1 - f[:, 0]
+        ix = (i0[:, 0] + dx) % n
+        for dy in (0, 1):
+            wy = f[:, 1] if dy else 1 - f[:, 1]
+            iy = (i0[:, 1] + dy) % n
+            for dz in (0, 1):
+                wz = f[:, 2] if dz else 1 - f[:, 2]
+                iz = (i0[:, 2] + dz) % n
+                out += wx * wy * wz * field[ix, iy, iz]
+    return out
 
 
 def amr_density(root: int = 4, n_cell: int = 16, levels: int = 3, sigma: float = 0.5,
-                seed: int = 7, threshold: float = 0.5, field_n: int = 256,
+                seed: int = 7, threshold: float = 0.5, field_n: int = 112,
                 r_soft: float = 0.02, extra_sigma: float = 0.0, extra_seed: int = 11):
     """Criterion-refined block mesh and the density rho ~ r^-2 * lognormal on it.
 
     A block at level l is refined when max over its cells of
     |grad rho| * dx / rho exceeds ``threshold`` (central differences of the
-    analytic density on that block's cell centres), 2:1 balance kept by the
-    tree."""
+    density sampled at that block's cell centres), 2:1 balance kept by the
+    tree.  The random field lives on a periodic field_n^3 grid (its k cutoff)
+    and is interpolated trilinearly.  Returns (mesh, rho at cell centres,
+    rho as a function of position)."""
     grf = gaussian_random_field(field_n, seed)
     grf2 = gaussian_random_field(field_n, extra_seed) if extra_sigma > 0 else None
 
@@ -123,19 +139,28 @@ def amr_density(root: int = 4, n_cell: int = 16, levels: int = 3, sigma: float =
         mesh.refine(lvl, np.array(kids, dtype=np.int32))
     mesh.finalize()
     rho = rho_at(mesh.cell_centers())
-    return mesh, rho
+    return mesh, rho, rho_at
+
+
+def centre_tau_norm(rho_at, n: int = 4096) -> float:
+    """Mean over the six axis directions of the column of rho from the domain
+    centre to the edge (midpoint rule on n points)."""
+    t = (np.arange(n) + 0.5) / n * 0.5
+    cols = []
+    for a in range(3):
+        for s in (1.0, -1.0):
+            x = np.full((n, 3), 0.5)
+            x[:, a] += s * t
+            cols.append(np.sum(rho_at(x)) * (0.5 / n))
+    return float(np.mean(cols))
 
 
 def config2(n_packets: int = 100_000_000, precision: int = _abi.KP_F32, seed: int = 42,
-            tau_scale: float = 10.0, **kw) -> Workload:
-    mesh, rho = amr_density(**kw)
-    # normalise so the centre-to-edge optical depth along +x is ~tau_scale
-    xc = mesh.cell_centers()
-    on_axis = (np.abs(xc[:, 1] - 0.5) < 0.02) & (np.abs(xc[:, 2] - 0.5) < 0.02) & (xc[:, 0] > 0.5)
-    dens = rho.copy()
-    lvl_h = np.repeat(mesh.dx0[:, 0], mesh.n3)
-    col = np.sum(dens[on_axis] * lvl_h[on_axis]) / max(1, int(round(0.04 / lvl_h[on_axis].mean()))) ** 2
-    kappa = (dens * (tau_scale / max(col, 1e-30))).astype(np.float32)
+            tau_centre: float = 5.0, **kw) -> Workload:
+    """kappa_rho = kappa0 * rho with kappa0 set so that the mean centre-to-edge
+    optical depth along the six axes is ``tau_centre``."""
+    mesh, rho, rho_at = amr_density(**kw)
+    kappa = (rho * (tau_centre / centre_tau_norm(rho_at))).astype(np.float32)
     albedo = np.full(mesh.n_cells, 0.5, dtype=np.float32)
     g = np.full(mesh.n_cells, 0.6, dtype=np.float32)
     run = RunSpec(bands=[BandSpec(n_packets=n_packets, albedo=0.5, g=0.6)], seed=seed,

@@ -72,6 +72,9 @@ struct kparams {
     double *image;
     unsigned long long *counters;
     double *end_state;
+    // persistent schedule
+    unsigned long long *work; // slot counter, zeroed before the launch
+    uint32_t chunk;           // slots a wave claims per refill
 };
 
 __device__ __forceinline__ void atomic_add_f64(double *p, double v)

@@ -16,8 +16,7 @@
 
 #include "kp_device.h"
 #include "kp_error.h"
-#include "kp_transport32.h"
-#include "kp_transport64.h"
+#include "kp_persist.h"
 
 #define KP_HIP(call)                                                                        \
     do {                                                                                    \
@@ -48,8 +47,9 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    int block_threads = 256;
-    int grid_blocks = 0; // 0: one thread per packet
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64};
+    int n_cu = 0;
+    unsigned long long *work = nullptr; // persistent slot counter
 };
 
 namespace {
@@ -145,6 +145,15 @@ __global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
     flush_counters(P, C);
 }
 
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+template <class Pol> __global__ void __launch_bounds__(256) k_persist(kp::kparams P)
+{
+    kp::lane_counters C;
+    kp::persist_body<Pol>(P, P.work, C);
+    flush_counters(P, C);
+}
+
 void free_all(kp_engine *e)
 {
     dfree(e->xf0);
@@ -162,6 +171,7 @@ void free_all(kp_engine *e)
     dfree(e->g);
     dfree(e->bands_d);
     dfree(e->band_start_d);
+    dfree(e->work);
     if (e->stage)
         (void)hipHostFree(e->stage);
     e->stage = nullptr;
@@ -279,6 +289,8 @@ kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **ou
             upload(e->nb_shift, m->nb_shift, 24 * nb);
             upload(e->d_lvl, m->d_lvl, 6 * nb);
             upload(e->phybnd, m->phybnd, nb);
+            e->work = dmalloc<unsigned long long>(1);
+            KP_HIP(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device));
             std::vector<float> h(3 * nb), o(3 * nb), v(nb);
             for (int b = 0; b < nb; ++b) {
                 for (int a = 0; a < 3; ++a) {
@@ -434,17 +446,58 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         P.end_state = out->end_state;
         if (total == 0)
             return;
-        const int T = e->block_threads;
-        uint64_t blocks = (total + T - 1) / T;
-        if (e->grid_blocks > 0 && (uint64_t)e->grid_blocks < blocks)
-            blocks = e->grid_blocks;
-        if (blocks > 0x7fffffffull)
-            blocks = 0x7fffffffull;
-        if (run->precision == KP_F64)
-            hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)blocks), dim3(T), 0, stream, P);
-        else
-            hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)blocks), dim3(T), 0, stream, P);
+        const kp_schedule &S = e->sched;
+        if (S.kind == KP_SCHED_HISTORY) {
+            const int T = 256;
+            uint64_t blocks = (total + T - 1) / T;
+            if (blocks > 0x7fffffffull)
+                blocks = 0x7fffffffull;
+            if (run->precision == KP_F64)
+                hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)blocks), dim3(T), 0, stream, P);
+            else
+                hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)blocks), dim3(T), 0, stream, P);
+        } else {
+            P.work = e->work;
+            P.chunk = (uint32_t)S.chunk;
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
+            uint64_t blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
+            const uint64_t need = (total + S.block_threads - 1) / S.block_threads;
+            if (blocks > need)
+                blocks = need;
+            if (run->precision == KP_F64)
+                hipLaunchKernelGGL(k_persist<kp::pol64>, dim3((unsigned)blocks),
+                                   dim3(S.block_threads), 0, stream, P);
+            else
+                hipLaunchKernelGGL(k_persist<kp::pol32>, dim3((unsigned)blocks),
+                                   dim3(S.block_threads), 0, stream, P);
+        }
         KP_HIP(hipGetLastError());
+    });
+}
+
+kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
+{
+    return kp::guard([&] {
+        if (!e || !s)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: null argument");
+        if (s->kind != KP_SCHED_HISTORY && s->kind != KP_SCHED_PERSISTENT)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: unknown kind");
+        if (s->block_threads != 64 && s->block_threads != 128 && s->block_threads != 256)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: block_threads must be 64, 128 or 256");
+        if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: blocks_per_cu must be in [1,64]");
+        if (s->chunk < 1 || s->chunk > 4096)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: chunk must be in [1,4096]");
+        e->sched = *s;
+    });
+}
+
+kp_status kp_get_schedule(const kp_engine *e, kp_schedule *s)
+{
+    return kp::guard([&] {
+        if (!e || !s)
+            throw kp::error(KP_ERR_ARG, "kp_get_schedule: null argument");
+        *s = e->sched;
     });
 }
 

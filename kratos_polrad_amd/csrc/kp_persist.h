@@ -1,0 +1,504 @@
+// kp_persist.h -- persistent transport kernel: one march step per iteration.
+//
+// The reference runs each packet's whole history in one thread of one
+// launch (integrator.h:80-106 -> photon.h:104-125).  Here every lane runs a
+// small state machine instead:
+//   NEED  - take the next packet slot from the wave's slot range (the wave
+//           claims CHUNK slots per global atomic, handed to lanes by ballot
+//           rank), emit it;
+//   MAIN  - one crossing of the packet's own path (tally, optical depth);
+//   PEEL  - one crossing of a peel-off ray toward an observer;
+//   DONE  - no work left.
+// Main-path and peel-off crossings execute the same face-distance and
+// block-change code, so a wave stays converged on the crossing loop while
+// its lanes are at different points of different histories; emission and
+// interactions are the only divergent branches.  The arithmetic of each
+// history is exactly that of the per-packet functions of the precision
+// policy (bit-identical to the CPU oracle in FP64).
+#pragma once
+
+#include "kp_transport32.h"
+#include "kp_transport64.h"
+
+namespace kp {
+
+enum : int { M_NEED = 0, M_MAIN = 1, M_PEEL = 2, M_DONE = 3 };
+enum : int { EV_EMIT = 0, EV_SCATTER = 1 };
+
+// ---------------------------------------------------------------- policies
+struct pol64 {
+    using real = double;
+    using ray = f64::ray_t;
+    __device__ static void at_source(const kparams &P, ray &r)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            r.x[a] = P.src_x[a];
+            r.i[a] = P.src_i[a];
+        }
+        r.ib = P.src_ib;
+        f64::gl_load(P, r);
+    }
+    __device__ static void relocate(const kparams &P, ray &r) { f64::gl_load(P, r); }
+    __device__ static void set_dir(const kparams &, ray &, const real *) {}
+    __device__ static int dl(const ray &r, real &l) { return f64::get_dl(r, l); }
+    __device__ static int cross(const kparams &P, ray &r, int ap, real l)
+    {
+        bool rl;
+        const int st = f64::cross_face(P, r, ap, l, rl);
+        if (st == f64::ST_IN && rl)
+            f64::gl_load(P, r);
+        return st;
+    }
+    __device__ static void advance(ray &r, real l)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            r.x[a] = r.x[a] + l * r.d[a];
+    }
+    __device__ static void gpos(const kparams &, const ray &r, real *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            g[a] = r.x[a];
+    }
+    __device__ static void gpos_d(const kparams &, const ray &r, double *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            g[a] = r.x[a];
+    }
+    __device__ static int cell(const kparams &P, const ray &r) { return f64::cell_index(P, r); }
+};
+
+struct pol32 {
+    using real = float;
+    using ray = f32::ray_t;
+    __device__ static void at_source(const kparams &P, ray &r)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            r.x[a] = P.src_xl[a];
+            r.i[a] = P.src_i[a];
+        }
+        r.ib = P.src_ib;
+        f32::set_scales(P, r);
+    }
+    __device__ static void relocate(const kparams &P, ray &r) { f32::set_scales(P, r); }
+    __device__ static void set_dir(const kparams &P, ray &r, const real *) { f32::set_scales(P, r); }
+    __device__ static int dl(const ray &r, real &l) { return f32::get_dl(r, l); }
+    __device__ static int cross(const kparams &P, ray &r, int ap, real l)
+    {
+        return f32::cross_face(P, r, ap, l);
+    }
+    __device__ static void advance(ray &r, real l)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            r.x[a] = fmaf(l, r.cpl[a], r.x[a]);
+    }
+    __device__ static void gpos(const kparams &P, const ray &r, real *g)
+    {
+        f32::global_pos(P, r, g);
+    }
+    __device__ static void gpos_d(const kparams &P, const ray &r, double *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            g[a] = (double)P.org32[3 * r.ib + a] + (double)r.x[a] * (double)P.h32[3 * r.ib + a];
+    }
+    __device__ static int cell(const kparams &P, const ray &r) { return f32::cell_index(P, r); }
+};
+
+// FP32 observers: loaded per use from the launch parameters
+template <class R> __device__ __forceinline__ void load3(const double *s, R *d)
+{
+    d[0] = (R)s[0];
+    d[1] = (R)s[1];
+    d[2] = (R)s[2];
+}
+
+// ---------------------------------------------------------------- kernel
+template <class Pol> struct lane_t {
+    using real = typename Pol::real;
+    using ray = typename Pol::ray;
+    ray cur;          // ray being marched (main path or peel-off)
+    real sx[3];       // main path saved while peel-off rays run
+    real sd[3];
+    int si[3];
+    int sib;
+    real e1[3];
+    real I, Q, U;     // packet Stokes vector
+    real Pi, Pq, Pu;  // Stokes vector of the current peel-off ray
+    double tau;       // MAIN: optical depth left; PEEL: accumulated
+    double tau_main;
+    real gg;          // HG asymmetry at the current interaction
+    uint32_t u[4];    // Philox words of the current interaction
+    uint64_t slot, pid;
+    int band;
+    int mode, ev_kind, o;
+    int events, steps;
+    uint32_t ncross, npeel;
+    f32::tally_acc acc; // FP32: Kahan register accumulator of the current cell
+};
+
+struct lane_counters {
+    uint32_t cross = 0, peel = 0, events = 0, escaped = 0, rr = 0, steps = 0, lost = 0,
+             packets = 0;
+};
+
+template <class Pol>
+__device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_counters &C,
+                                       int status)
+{
+    if constexpr (sizeof(typename Pol::real) == 4)
+        L.acc.flush(P, L.band);
+    C.cross += L.ncross;
+    C.peel += L.npeel;
+    C.events += (uint32_t)L.events;
+    C.packets += 1;
+    if (status == KP_END_ESCAPED)
+        C.escaped += 1;
+    else if (status == KP_END_RR)
+        C.rr += 1;
+    else if (status == KP_END_STEPS)
+        C.steps += 1;
+    else
+        C.lost += 1;
+    if (P.end_state) {
+        double *es = P.end_state + L.slot * KP_ES_STRIDE;
+        double g[3];
+        Pol::gpos_d(P, L.cur, g);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            es[KP_ES_X + a] = g[a];
+            es[KP_ES_D + a] = (double)L.cur.d[a];
+        }
+        es[KP_ES_I] = (double)L.I;
+        es[KP_ES_Q] = (double)L.Q;
+        es[KP_ES_U] = (double)L.U;
+        es[KP_ES_EVENTS] = (double)L.events;
+        es[KP_ES_CROSS] = (double)L.ncross;
+        es[KP_ES_STATUS] = (double)status;
+        es[KP_ES_IB] = (double)L.cur.ib;
+        es[KP_ES_CELL] = (double)Pol::cell(P, L.cur);
+        es[14] = (double)L.npeel;
+        es[15] = 0.0;
+    }
+    L.mode = M_NEED;
+}
+
+// Start the peel-off ray toward observer L.o from the saved main position.
+template <class Pol> __device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L)
+{
+    using real = typename Pol::real;
+    real no[3], ev[3];
+    load3(P.no[L.o], no);
+    load3(P.ev[L.o], ev);
+    if (L.ev_kind == EV_EMIT) {
+        L.Pi = L.I;
+        L.Pq = 0;
+        L.Pu = 0;
+    } else {
+        real e2[3];
+        if constexpr (sizeof(real) == 8) {
+            f64::cross3(L.sd, L.e1, e2);
+            f64::peel_stokes(P.model, L.gg, P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no, ev, L.Pi,
+                             L.Pq, L.Pu);
+        } else {
+            f32::cross3(L.sd, L.e1, e2);
+            f32::peel_stokes(P.model, L.gg, (float)P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no, ev,
+                             L.Pi, L.Pq, L.Pu);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        L.cur.x[a] = L.sx[a];
+        L.cur.i[a] = L.si[a];
+        L.cur.d[a] = no[a];
+    }
+    L.cur.ib = L.sib;
+    Pol::relocate(P, L.cur);
+    L.tau = 0.0;
+    L.mode = M_PEEL;
+}
+
+template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        L.sx[a] = L.cur.x[a];
+        L.sd[a] = L.cur.d[a];
+        L.si[a] = L.cur.i[a];
+    }
+    L.sib = L.cur.ib;
+    L.tau_main = L.tau;
+}
+
+// After the interaction's peel-offs: scatter, new free flight, roulette.
+template <class Pol>
+__device__ __forceinline__ void finish_event(const kparams &P, lane_t<Pol> &L, lane_counters &C)
+{
+    using real = typename Pol::real;
+    real e2[3];
+    if constexpr (sizeof(real) == 8) {
+        f64::cross3(L.cur.d, L.e1, e2);
+        const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(L.u[0]))
+                                                  : kp_hg_mu(L.gg, kp_u01(L.u[0]));
+        f64::scatter_stokes(P.model, P.pmax, mu, kp_u01(L.u[1]), L.cur.d, L.e1, e2, L.I, L.Q,
+                            L.U);
+        L.tau = -kp_log(kp_u01(L.u[2]));
+        if (L.I < P.rr_w * P.w0) {
+            if (kp_u01(L.u[3]) < P.rr_s) {
+                L.I = L.I / P.rr_s;
+                L.Q = L.Q / P.rr_s;
+                L.U = L.U / P.rr_s;
+            } else {
+                finish(P, L, C, KP_END_RR);
+                return;
+            }
+        }
+    } else {
+        f32::cross3(L.cur.d, L.e1, e2);
+        const float xi0 = f32::u01(L.u[0]);
+        const float mu = (P.model == KP_THOMSON) ? f32::thomson_mu(xi0) : f32::hg_mu(L.gg, xi0);
+        f32::scatter_stokes(P.model, (float)P.pmax, mu, f32::u01(L.u[1]), L.cur.d, L.e1, e2, L.I,
+                            L.Q, L.U);
+        Pol::set_dir(P, L.cur, L.cur.d);
+        L.tau = -(double)f32::fast_log(f32::u01(L.u[2]));
+        if (L.I < (float)P.rr_w * (float)P.w0) {
+            if (f32::u01(L.u[3]) < (float)P.rr_s) {
+                const float inv = (float)(1.0 / P.rr_s);
+                L.I *= inv;
+                L.Q *= inv;
+                L.U *= inv;
+            } else {
+                finish(P, L, C, KP_END_RR);
+                return;
+            }
+        }
+    }
+    L.mode = M_MAIN;
+}
+
+template <class Pol>
+__device__ __forceinline__ void end_peel(const kparams &P, lane_t<Pol> &L, lane_counters &C)
+{
+    using real = typename Pol::real;
+    // deposit at the projected interaction (or emission) point
+    if (P.image) {
+        if constexpr (sizeof(real) == 8) {
+            const double att = kp_exp(-L.tau) * KP_INV_4PI;
+            f64::deposit(P, L.band, L.o, L.sx, L.Pi * att, L.Pq * att, L.Pu * att);
+        } else {
+            f32::ray_t tmp;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                tmp.x[a] = L.sx[a];
+            tmp.ib = L.sib;
+            float gx[3];
+            f32::global_pos(P, tmp, gx);
+            const float att = f32::fast_exp(-(float)L.tau) * f32::kInv4Pi;
+            f32::deposit(P, L.band, L.o, gx, L.Pi * att, L.Pq * att, L.Pu * att);
+        }
+    }
+    L.o += 1;
+    if (L.o < P.n_incl) {
+        start_peel<Pol>(P, L);
+        return;
+    }
+    // resume the main path
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        L.cur.x[a] = L.sx[a];
+        L.cur.d[a] = L.sd[a];
+        L.cur.i[a] = L.si[a];
+    }
+    L.cur.ib = L.sib;
+    Pol::relocate(P, L.cur);
+    L.tau = L.tau_main;
+    if (L.ev_kind == EV_EMIT)
+        L.mode = M_MAIN;
+    else
+        finish_event<Pol>(P, L, C);
+}
+
+template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
+{
+    L.band = band_of(P, L.slot);
+    L.pid = P.packet_offset + (L.slot - P.band_start[L.band]);
+    uint32_t rn[4] = {(uint32_t)L.pid, (uint32_t)(L.pid >> 32), 0u, 0u};
+    kp_philox4x32_10(rn, P.seed, (uint32_t)L.band);
+    if constexpr (sizeof(typename Pol::real) == 8)
+        f64::emit(rn, L.cur.d, L.e1, L.tau);
+    else
+        f32::emit(rn, L.cur.d, L.e1, L.tau);
+    L.I = (typename Pol::real)P.w0;
+    L.Q = 0;
+    L.U = 0;
+    Pol::at_source(P, L.cur);
+    L.events = 0;
+    L.steps = P.step_max;
+    L.ncross = 0;
+    L.npeel = 0;
+    if (P.n_incl > 0) {
+        save_main(L);
+        L.ev_kind = EV_EMIT;
+        L.o = 0;
+        start_peel<Pol>(P, L);
+    } else {
+        L.mode = M_MAIN;
+    }
+}
+
+template <class Pol>
+__device__ __forceinline__ void interact(const kparams &P, lane_t<Pol> &L, lane_counters &C,
+                                         int64_t gc)
+{
+    using real = typename Pol::real;
+    L.events += 1;
+    L.u[0] = (uint32_t)L.pid;
+    L.u[1] = (uint32_t)(L.pid >> 32);
+    L.u[2] = (uint32_t)L.events;
+    L.u[3] = 0u;
+    kp_philox4x32_10(L.u, P.seed, (uint32_t)L.band);
+    const dband &bd = P.bands[L.band];
+    real alb, gg;
+    if constexpr (sizeof(real) == 8) {
+        alb = (bd.flags & KP_BAND_CELL_ALBEDO) ? (double)P.albedo[gc] : bd.albedo;
+        gg = (bd.flags & KP_BAND_CELL_G) ? (P.g ? (double)P.g[gc] : 0.0) : bd.g;
+        L.I = L.I * alb;
+        L.Q = L.Q * alb;
+        L.U = L.U * alb;
+    } else {
+        alb = (bd.flags & KP_BAND_CELL_ALBEDO) ? P.albedo[gc] : (float)bd.albedo;
+        gg = (bd.flags & KP_BAND_CELL_G) ? (P.g ? P.g[gc] : 0.0f) : (float)bd.g;
+        L.I *= alb;
+        L.Q *= alb;
+        L.U *= alb;
+    }
+    L.gg = gg;
+    if (P.n_incl > 0) {
+        save_main(L);
+        L.ev_kind = EV_SCATTER;
+        L.o = 0;
+        start_peel<Pol>(P, L);
+    } else {
+        finish_event<Pol>(P, L, C);
+    }
+}
+
+template <class Pol>
+__device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C)
+{
+    using real = typename Pol::real;
+    lane_t<Pol> L;
+    L.mode = M_NEED;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t wbase = 0, wend = 0; // wave-uniform slot range
+
+    for (;;) {
+        // ---- refill idle lanes from the wave's slot range
+        const uint64_t need = __ballot(L.mode == M_NEED);
+        if (need) {
+            const uint32_t n = __popcll(need);
+            const uint32_t rank = __popcll(need & lt_mask);
+            const uint64_t avail = wend - wbase;
+            uint64_t my = wbase + rank;
+            if (n > avail) {
+                uint64_t nb = 0;
+                if (lane == __ffsll((long long)need) - 1)
+                    nb = atomicAdd(work, (unsigned long long)P.chunk);
+                nb = __shfl(nb, __ffsll((long long)need) - 1, 64);
+                if (rank >= avail)
+                    my = nb + (rank - avail);
+                wbase = nb + (n - avail);
+                wend = nb + P.chunk;
+            } else {
+                wbase += n;
+            }
+            if (L.mode == M_NEED) {
+                if (my < P.total) {
+                    L.slot = my;
+                    emit_packet<Pol>(P, L);
+                } else {
+                    L.mode = M_DONE;
+                }
+            }
+        }
+        if (!__ballot(L.mode != M_DONE))
+            break;
+        if (L.mode != M_MAIN && L.mode != M_PEEL)
+            continue;
+
+        // ---- one march step of the current ray
+        const bool is_main = (L.mode == M_MAIN);
+        if (is_main && L.steps <= 0) {
+            finish(P, L, C, KP_END_STEPS);
+            continue;
+        }
+        real l;
+        const int ap = Pol::dl(L.cur, l);
+        if (ap < 0) {
+            if (is_main) {
+                finish(P, L, C, KP_END_LOST);
+            } else {
+                end_peel<Pol>(P, L, C);
+            }
+            continue;
+        }
+        const int64_t gc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
+        const real ks = (real)P.bands[L.band].kappa_scale;
+        const real kap = (real)P.kappa[gc] * ks;
+        const real dtau = kap * l;
+        if (is_main) {
+            L.ncross += 1;
+            L.steps -= 1;
+            if constexpr (sizeof(real) == 8) {
+                const double V = P.vol0[L.cur.ib];
+                if (kap > 0.0 && dtau >= L.tau) {
+                    double li = L.tau / kap;
+                    if (li > l)
+                        li = l;
+                    if (P.J)
+                        atomic_add_f64(P.J + gc * P.n_bands + L.band, L.I * (li / V));
+                    Pol::advance(L.cur, li);
+                    interact<Pol>(P, L, C, gc);
+                    continue;
+                }
+                L.tau = L.tau - dtau;
+                if (P.J)
+                    atomic_add_f64(P.J + gc * P.n_bands + L.band, L.I * (l / V));
+            } else {
+                const float iV = f32::fast_rcp(P.vol32[L.cur.ib]);
+                if (kap > 0.0f && (double)dtau >= L.tau) {
+                    const float li = fminf((float)L.tau * f32::fast_rcp(kap), l);
+                    L.acc.add(P, L.band, gc, L.I * li * iV);
+                    Pol::advance(L.cur, li);
+                    interact<Pol>(P, L, C, gc);
+                    continue;
+                }
+                L.tau -= (double)dtau;
+                L.acc.add(P, L.band, gc, L.I * l * iV);
+            }
+        } else {
+            L.tau = L.tau + (double)dtau;
+            L.npeel += 1;
+            if (L.tau > kTauCut) {
+                end_peel<Pol>(P, L, C);
+                continue;
+            }
+        }
+        const int st = Pol::cross(P, L.cur, ap, l);
+        if (st != f64::ST_IN) {
+            if (is_main) {
+                finish(P, L, C, st == f64::ST_ESCAPED ? KP_END_ESCAPED : KP_END_LOST);
+            } else {
+                end_peel<Pol>(P, L, C);
+            }
+        }
+    }
+}
+
+} // namespace kp

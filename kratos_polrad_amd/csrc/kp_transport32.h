@@ -105,48 +105,53 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, fl
         }
     }
     const int a = ap;
-    if (r.i[a] >= 0 && r.i[a] < n)
+    // runtime axis indices are resolved with unrolled selects so the ray
+    // stays in registers
+    const int ia = a == 0 ? r.i[0] : (a == 1 ? r.i[1] : r.i[2]);
+    if (ia >= 0 && ia < n)
         return ST_IN;
-    const int side = r.i[a] > 0;
+    const int side = ia > 0;
     if (P.phybnd[r.ib] & (1u << (2 * a + side)))
         return ST_ESCAPED;
     const int ap0 = a == 2 ? 0 : a + 1;
-    const int ap1 = a == 0 ? 2 : a - 1;
     const int dl = P.d_lvl[6 * r.ib + 2 * a + side];
     int j0 = 0, j1 = 0;
     if (dl > 0) {
-        j0 = r.i[ap0] >= n / 2;
-        j1 = r.i[ap1] >= n / 2;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (q == a)
+                continue;
+            const int jj = r.i[q] >= n / 2;
+            if (q == ap0)
+                j0 = jj;
+            else
+                j1 = jj;
+        }
     }
     const int slot = 24 * r.ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
     const int nb = P.nb_id[slot];
     if (nb < 0)
         return ST_LOST;
+    const int shift = P.nb_shift[slot];
     r.ib = nb;
-    r.i[a] = side ? 0 : n - 1;
-    r.x[a] = side ? 0.0f : nf;
     const float half = (float)(n / 2);
-    if (dl < 0) {
-        const int shift = P.nb_shift[slot];
-        r.x[ap0] = fmaf(r.x[ap0], 0.5f, (float)(shift & 1) * half);
-        r.x[ap1] = fmaf(r.x[ap1], 0.5f, (float)((shift >> 1) & 1) * half);
+    const float sc = dl < 0 ? 2.0f : (dl > 0 ? 0.5f : 1.0f);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            r.lpc[q] *= 2.0f;
-            r.cpl[q] *= 0.5f;
+    for (int q = 0; q < 3; ++q) {
+        r.lpc[q] *= sc;
+        r.cpl[q] *= 1.0f / sc;
+        if (q == a) {
+            r.i[q] = side ? 0 : n - 1;
+            r.x[q] = side ? 0.0f : nf;
+            continue;
         }
-    } else if (dl > 0) {
-        r.x[ap0] = (r.x[ap0] - (float)j0 * half) * 2.0f;
-        r.x[ap1] = (r.x[ap1] - (float)j1 * half) * 2.0f;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            r.lpc[q] *= 0.5f;
-            r.cpl[q] *= 2.0f;
-        }
-    }
-    if (dl != 0) {
-        r.i[ap0] = min(max((int)floorf(r.x[ap0]), 0), n - 1);
-        r.i[ap1] = min(max((int)floorf(r.x[ap1]), 0), n - 1);
+        const int j = (q == ap0) ? 0 : 1;
+        if (dl < 0)
+            r.x[q] = fmaf(r.x[q], 0.5f, (float)((shift >> j) & 1) * half);
+        else if (dl > 0)
+            r.x[q] = (r.x[q] - (float)(j == 0 ? j0 : j1) * half) * 2.0f;
+        if (dl != 0)
+            r.i[q] = min(max((int)floorf(r.x[q]), 0), n - 1);
     }
     return ST_IN;
 }
@@ -257,6 +262,111 @@ __device__ __forceinline__ float hg_mu(float g, float xi)
         mu = (1.0f + g2 - t * t) * fast_rcp(2.0f * g);
     }
     return fminf(fmaxf(mu, -1.0f), 1.0f);
+}
+
+// Peel-off Stokes vector toward no in the image frame (Q>0 along ev).
+__device__ __forceinline__ void peel_stokes(int model, float gg, float pmax, const float *d,
+                                            const float *e1, const float *e2, float I, float Q,
+                                            float U, const float *no, const float *ev,
+                                            float &Io, float &Qo, float &Uo)
+{
+    const float mo = fminf(fmaxf(dot3(d, no), -1.0f), 1.0f);
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        p[a] = fmaf(-mo, d[a], no[a]);
+    const float pn2 = dot3(p, p);
+    float cf = 1.0f, sf = 0.0f, st = 0.0f;
+    if (pn2 > 1e-12f) {
+        const float ip = fast_rsq(pn2);
+        cf = dot3(p, e1) * ip;
+        sf = dot3(p, e2) * ip;
+        st = pn2 * ip;
+    }
+    const float c2 = fmaf(cf, cf, -sf * sf);
+    const float s2 = 2.0f * cf * sf;
+    const float Qr = fmaf(Q, c2, U * s2);
+    const float Ur = fmaf(U, c2, -Q * s2);
+    float z11, z12, z33;
+    phase(model, gg, pmax, mo, z11, z12, z33);
+    const float Ip = fmaf(z11, I, z12 * Qr);
+    const float Qp = fmaf(z12, I, z11 * Qr);
+    const float Up = z33 * Ur;
+    float e1o[3], e2o[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float e1r = fmaf(cf, e1[a], sf * e2[a]);
+        e1o[a] = fmaf(mo, e1r, -st * d[a]);
+    }
+    cross3(no, e1o, e2o);
+    const float cps = dot3(ev, e1o);
+    const float sps = dot3(ev, e2o);
+    const float inrm = fast_rcp(fmaf(cps, cps, sps * sps));
+    const float c2p = fmaf(cps, cps, -sps * sps) * inrm;
+    const float s2p = 2.0f * cps * sps * inrm;
+    Io = Ip;
+    Qo = fmaf(Qp, c2p, Up * s2p);
+    Uo = fmaf(Up, c2p, -Qp * s2p);
+}
+
+// One scattering by (mu, azimuth 2 pi phi_u from e1): new d, e1, Stokes weight.
+__device__ __forceinline__ void scatter_stokes(int model, float pmax, float mu, float phi_u,
+                                               float *d, float *e1, const float *e2, float &I,
+                                               float &Q, float &U)
+{
+    const float st = fast_sqrt(fmaxf(fmaf(-mu, mu, 1.0f), 0.0f));
+    const float sf = __builtin_amdgcn_sinf(phi_u), cf = __builtin_amdgcn_cosf(phi_u);
+    const float c2 = fmaf(cf, cf, -sf * sf);
+    const float s2 = 2.0f * cf * sf;
+    const float Qr = fmaf(Q, c2, U * s2);
+    const float Ur = fmaf(U, c2, -Q * s2);
+    const float m2 = mu * mu;
+    const float pl = (model == KP_THOMSON) ? 1.0f : pmax;
+    const float r1 = fast_rcp(1.0f + m2);
+    const float a12 = -pl * (1.0f - m2) * r1;
+    const float a33 = 2.0f * mu * r1;
+    const float In = fmaf(a12, Qr, I);
+    const float Qn = fmaf(a12, I, Qr);
+    const float Un = a33 * Ur;
+    I = In;
+    Q = Qn;
+    U = Un;
+    float dn[3], en[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float e1r = fmaf(cf, e1[a], sf * e2[a]);
+        dn[a] = fmaf(mu, d[a], st * e1r);
+        en[a] = fmaf(mu, e1r, -st * d[a]);
+    }
+    const float id = fast_rsq(dot3(dn, dn));
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        dn[a] *= id;
+    const float pr = dot3(en, dn);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        en[a] = fmaf(-pr, dn[a], en[a]);
+    const float ie = fast_rsq(dot3(en, en));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d[a] = dn[a];
+        e1[a] = en[a] * ie;
+    }
+}
+
+__device__ __forceinline__ void emit(const uint32_t rn[4], float *d, float *e1, double &tau)
+{
+    const float mu = fmaf(2.0f, u01(rn[0]), -1.0f);
+    const float ph = u01(rn[1]);
+    const float sf = __builtin_amdgcn_sinf(ph), cf = __builtin_amdgcn_cosf(ph);
+    const float st = fast_sqrt(fmaxf(fmaf(-mu, mu, 1.0f), 0.0f));
+    d[0] = st * cf;
+    d[1] = st * sf;
+    d[2] = mu;
+    e1[0] = mu * cf;
+    e1[1] = mu * sf;
+    e1[2] = -st;
+    tau = -(double)fast_log(u01(rn[2]));
 }
 
 // Kahan-compensated register accumulator for one cell of the J tally.

@@ -101,17 +101,27 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, do
     }
     const int n = P.n_cell;
     const int a = ap; // only the exit axis can leave [0,n)
-    if (!(r.i[a] >= 0 && r.i[a] < n)) {
-        const int side = r.i[a] > 0;
+    // runtime axis indices are resolved with unrolled selects so the ray
+    // stays in registers
+    const int ia = a == 0 ? r.i[0] : (a == 1 ? r.i[1] : r.i[2]);
+    if (!(ia >= 0 && ia < n)) {
+        const int side = ia > 0;
         if (P.phybnd[r.ib] & (1u << (2 * a + side)))
             return ST_ESCAPED;
         const int ap0 = a == 2 ? 0 : a + 1;
-        const int ap1 = a == 0 ? 2 : a - 1;
         const int dl = P.d_lvl[6 * r.ib + 2 * a + side];
         int j0 = 0, j1 = 0;
         if (dl > 0) {
-            j0 = r.i[ap0] >= n / 2;
-            j1 = r.i[ap1] >= n / 2;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (q == a)
+                    continue;
+                const int jj = r.i[q] >= n / 2;
+                if (q == ap0)
+                    j0 = jj;
+                else
+                    j1 = jj;
+            }
         }
         const int slot = 24 * r.ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
         const int nb = P.nb_id[slot];
@@ -119,19 +129,21 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, do
             return ST_LOST;
         const int shift = P.nb_shift[slot];
         r.ib = nb;
-        r.i[a] = side ? 0 : n - 1;
-        if (dl < 0) {
-            r.i[ap0] = r.i[ap0] / 2 + (shift & 1) * (n / 2);
-            r.i[ap1] = r.i[ap1] / 2 + ((shift >> 1) & 1) * (n / 2);
-        } else if (dl > 0) {
-            int f0 = (r.i[ap0] < n / 2 ? r.i[ap0] : r.i[ap0] - n / 2) * 2;
-            if (r.x[ap0] >= x_fc(P, nb, ap0, f0 + 1))
-                f0 += 1;
-            int f1 = (r.i[ap1] < n / 2 ? r.i[ap1] : r.i[ap1] - n / 2) * 2;
-            if (r.x[ap1] >= x_fc(P, nb, ap1, f1 + 1))
-                f1 += 1;
-            r.i[ap0] = f0;
-            r.i[ap1] = f1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (q == a) {
+                r.i[q] = side ? 0 : n - 1;
+                continue;
+            }
+            const int j = (q == ap0) ? 0 : 1;
+            if (dl < 0) {
+                r.i[q] = r.i[q] / 2 + ((shift >> j) & 1) * (n / 2);
+            } else if (dl > 0) {
+                int f = (r.i[q] < n / 2 ? r.i[q] : r.i[q] - n / 2) * 2;
+                if (r.x[q] >= x_fc(P, nb, q, f + 1))
+                    f += 1;
+                r.i[q] = f;
+            }
         }
         reload = true;
     }
@@ -223,6 +235,126 @@ __device__ __forceinline__ void phase(int model, double gg, double pmax, double 
     }
 }
 
+// Peel-off Stokes vector toward no in the image frame (Q>0 along ev), without
+// the 1/(4 pi) and attenuation factors.  Same operation order as the oracle.
+__device__ __forceinline__ void peel_stokes(int model, double gg, double pmax, const double *d,
+                                            const double *e1, const double *e2, double I,
+                                            double Q, double U, const double *no,
+                                            const double *ev, double &Io, double &Qo,
+                                            double &Uo)
+{
+    double mo = dot3(d, no);
+    if (mo > 1.0)
+        mo = 1.0;
+    if (mo < -1.0)
+        mo = -1.0;
+    double p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        p[a] = no[a] - mo * d[a];
+    const double pn2 = dot3(p, p);
+    double cf = 1.0, sf = 0.0, st = 0.0;
+    if (pn2 > 1e-24) {
+        const double ip = 1.0 / sqrt(pn2);
+        cf = dot3(p, e1) * ip;
+        sf = dot3(p, e2) * ip;
+        st = pn2 * ip;
+    }
+    const double c2 = cf * cf - sf * sf;
+    const double s2 = 2.0 * cf * sf;
+    const double Qr = Q * c2 + U * s2;
+    const double Ur = U * c2 - Q * s2;
+    double z11, z12, z33;
+    phase(model, gg, pmax, mo, z11, z12, z33);
+    const double Ip = z11 * I + z12 * Qr;
+    const double Qp = z12 * I + z11 * Qr;
+    const double Up = z33 * Ur;
+    double e1o[3], e2o[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double e1r = cf * e1[a] + sf * e2[a];
+        e1o[a] = mo * e1r - st * d[a];
+    }
+    cross3(no, e1o, e2o);
+    const double cps = dot3(ev, e1o);
+    const double sps = dot3(ev, e2o);
+    const double nrm = cps * cps + sps * sps;
+    const double c2p = (cps * cps - sps * sps) / nrm;
+    const double s2p = 2.0 * cps * sps / nrm;
+    Io = Ip;
+    Qo = Qp * c2p + Up * s2p;
+    Uo = Up * c2p - Qp * s2p;
+}
+
+// One scattering: new direction d and frame e1 in place, Stokes weight
+// Z S / Z11.  Same operation order as the oracle.
+__device__ __forceinline__ void scatter_stokes(int model, double pmax, double mu, double phi_u,
+                                               double *d, double *e1, const double *e2,
+                                               double &I, double &Q, double &U)
+{
+    double s2m = fma(-mu, mu, 1.0);
+    if (s2m < 0.0)
+        s2m = 0.0;
+    const double st = sqrt(s2m);
+    double sf, cf;
+    kp_sincos2pi(phi_u, &sf, &cf);
+    const double c2 = cf * cf - sf * sf;
+    const double s2 = 2.0 * cf * sf;
+    const double Qr = Q * c2 + U * s2;
+    const double Ur = U * c2 - Q * s2;
+    const double m2 = mu * mu;
+    const double pl = (model == KP_THOMSON) ? 1.0 : pmax;
+    const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
+    const double a33 = 2.0 * mu / (1.0 + m2);
+    const double In = I + a12 * Qr;
+    const double Qn = a12 * I + Qr;
+    const double Un = a33 * Ur;
+    I = In;
+    Q = Qn;
+    U = Un;
+    double dn[3], en[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double e1r = cf * e1[a] + sf * e2[a];
+        dn[a] = mu * d[a] + st * e1r;
+        en[a] = mu * e1r - st * d[a];
+    }
+    const double id = 1.0 / sqrt(dot3(dn, dn));
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        dn[a] = dn[a] * id;
+    const double pr = dot3(en, dn);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        en[a] = en[a] - pr * dn[a];
+    const double ie = 1.0 / sqrt(dot3(en, en));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d[a] = dn[a];
+        e1[a] = en[a] * ie;
+    }
+}
+
+// Emission (event 0): isotropic direction, theta-hat reference axis, first
+// free flight.
+__device__ __forceinline__ void emit(const uint32_t rn[4], double *d, double *e1, double &tau)
+{
+    const double mu = fma(2.0, kp_u01(rn[0]), -1.0);
+    double sf, cf;
+    kp_sincos2pi(kp_u01(rn[1]), &sf, &cf);
+    double s2 = fma(-mu, mu, 1.0);
+    if (s2 < 0.0)
+        s2 = 0.0;
+    const double st = sqrt(s2);
+    d[0] = st * cf;
+    d[1] = st * sf;
+    d[2] = mu;
+    e1[0] = mu * cf;
+    e1[1] = mu * sf;
+    e1[2] = -st;
+    tau = -kp_log(kp_u01(rn[2]));
+}
+
 struct counters_t {
     uint64_t cross = 0, peel = 0, events = 0, escaped = 0, rr = 0, steps = 0, lost = 0,
              packets = 0;
@@ -243,22 +375,8 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
     kp_philox4x32_10(rn, k0, k1);
     ray_t r;
     double e1[3];
-    {
-        const double mu = fma(2.0, kp_u01(rn[0]), -1.0);
-        double sf, cf;
-        kp_sincos2pi(kp_u01(rn[1]), &sf, &cf);
-        double s2 = fma(-mu, mu, 1.0);
-        if (s2 < 0.0)
-            s2 = 0.0;
-        const double st = sqrt(s2);
-        r.d[0] = st * cf;
-        r.d[1] = st * sf;
-        r.d[2] = mu;
-        e1[0] = mu * cf;
-        e1[1] = mu * sf;
-        e1[2] = -st;
-    }
-    double tau_rem = -kp_log(kp_u01(rn[2]));
+    double tau_rem;
+    emit(rn, r.d, e1, tau_rem);
     double I = w0, Q = 0.0, U = 0.0;
     r.x[0] = P.src_x[0];
     r.x[1] = P.src_x[1];
@@ -341,48 +459,9 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
         cross3(r.d, e1, e2);
 
         for (int o = 0; o < P.n_incl; ++o) {
-            const double *no = P.no[o];
-            double mo = dot3(r.d, no);
-            if (mo > 1.0)
-                mo = 1.0;
-            if (mo < -1.0)
-                mo = -1.0;
-            double p[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                p[a] = no[a] - mo * r.d[a];
-            const double pn2 = dot3(p, p);
-            double cf = 1.0, sf = 0.0, st = 0.0;
-            if (pn2 > 1e-24) {
-                const double ip = 1.0 / sqrt(pn2);
-                cf = dot3(p, e1) * ip;
-                sf = dot3(p, e2) * ip;
-                st = pn2 * ip;
-            }
-            const double c2 = cf * cf - sf * sf;
-            const double s2 = 2.0 * cf * sf;
-            const double Qr = Q * c2 + U * s2;
-            const double Ur = U * c2 - Q * s2;
-            double z11, z12, z33;
-            phase(P.model, gg, P.pmax, mo, z11, z12, z33);
-            const double Ip = z11 * I + z12 * Qr;
-            const double Qp = z12 * I + z11 * Qr;
-            const double Up = z33 * Ur;
-            double e1o[3], e2o[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const double e1r = cf * e1[a] + sf * e2[a];
-                e1o[a] = mo * e1r - st * r.d[a];
-            }
-            cross3(no, e1o, e2o);
-            const double cps = dot3(P.ev[o], e1o);
-            const double sps = dot3(P.ev[o], e2o);
-            const double nrm = cps * cps + sps * sps;
-            const double c2p = (cps * cps - sps * sps) / nrm;
-            const double s2p = 2.0 * cps * sps / nrm;
-            const double Qi = Qp * c2p + Up * s2p;
-            const double Ui = Up * c2p - Qp * s2p;
-            const double tau = tau_to_edge(P, r, no, ks, npeel);
+            double Ip, Qi, Ui;
+            peel_stokes(P.model, gg, P.pmax, r.d, e1, e2, I, Q, U, P.no[o], P.ev[o], Ip, Qi, Ui);
+            const double tau = tau_to_edge(P, r, P.no[o], ks, npeel);
             const double att = kp_exp(-tau) * KP_INV_4PI;
             deposit(P, band, o, r.x, Ip * att, Qi * att, Ui * att);
         }
@@ -390,47 +469,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
         {
             const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
                                                       : kp_hg_mu(gg, kp_u01(u[0]));
-            double s2m = fma(-mu, mu, 1.0);
-            if (s2m < 0.0)
-                s2m = 0.0;
-            const double st = sqrt(s2m);
-            double sf, cf;
-            kp_sincos2pi(kp_u01(u[1]), &sf, &cf);
-            const double c2 = cf * cf - sf * sf;
-            const double s2 = 2.0 * cf * sf;
-            const double Qr = Q * c2 + U * s2;
-            const double Ur = U * c2 - Q * s2;
-            const double m2 = mu * mu;
-            const double pl = (P.model == KP_THOMSON) ? 1.0 : P.pmax;
-            const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
-            const double a33 = 2.0 * mu / (1.0 + m2);
-            const double In = I + a12 * Qr;
-            const double Qn = a12 * I + Qr;
-            const double Un = a33 * Ur;
-            I = In;
-            Q = Qn;
-            U = Un;
-            double dn[3], en[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const double e1r = cf * e1[a] + sf * e2[a];
-                dn[a] = mu * r.d[a] + st * e1r;
-                en[a] = mu * e1r - st * r.d[a];
-            }
-            const double id = 1.0 / sqrt(dot3(dn, dn));
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                dn[a] = dn[a] * id;
-            const double pr = dot3(en, dn);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                en[a] = en[a] - pr * dn[a];
-            const double ie = 1.0 / sqrt(dot3(en, en));
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                r.d[a] = dn[a];
-                e1[a] = en[a] * ie;
-            }
+            scatter_stokes(P.model, P.pmax, mu, kp_u01(u[1]), r.d, e1, e2, I, Q, U);
         }
         tau_rem = -kp_log(kp_u01(u[2]));
         if (I < P.rr_w * w0) {

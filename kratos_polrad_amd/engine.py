@@ -103,6 +103,16 @@ class Engine:
     def handle(self):
         return self._h
 
+    def set_schedule(self, kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4,
+                     chunk=64) -> None:
+        s = _abi.Schedule(int(kind), int(block_threads), int(blocks_per_cu), int(chunk))
+        _abi.check(_abi.lib().kp_set_schedule(self._h, C.byref(s)))
+
+    def get_schedule(self) -> dict:
+        s = _abi.Schedule()
+        _abi.check(_abi.lib().kp_get_schedule(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.Schedule._fields_}
+
     def set_media(self, kappa_rho, albedo, g=None) -> None:
         n = self.mesh.n_cells
         k = np.ascontiguousarray(np.asarray(kappa_rho, dtype=np.float32).reshape(n))

@@ -18,13 +18,19 @@ from kratos_polrad_amd.mesh import Mesh
 pytestmark = pytest.mark.gpu
 
 
-def _run_gpu(w, end_state=True, precision=None):
+SCHEDULES = [dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64),
+             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1)]
+
+
+def _run_gpu(w, end_state=True, precision=None, sched=None):
     import torch
 
     run = w.run
     if precision is not None:
         run = RunSpec(**{**run.__dict__, "precision": precision})
     eng = Engine(w.mesh)
+    if sched is not None:
+        eng.set_schedule(**sched)
     eng.set_media(w.kappa, w.albedo, w.g)
     out = eng.alloc_outputs(run, end_state=end_state)
     eng.forward(w.source, run, out)
@@ -64,9 +70,10 @@ def _amr_workload(n_packets, precision=_abi.KP_F64, model=_abi.KP_HG_RAYLEIGH):
     return configs.Workload("amr_test", mesh, kappa, albedo, g, (0.51, 0.49, 0.5), run)
 
 
-def test_f64_config1_bit_exact(oracle_lib):
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+def test_f64_config1_bit_exact(oracle_lib, sched):
     w = configs.config1(n_packets=20000)
-    got = _run_gpu(w)
+    got = _run_gpu(w, sched=sched)
     ref = oracle_lib.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, w.run, end_state=True)
     _compare_end_states(got["end_state"], ref["end_state"])
     assert np.array_equal(got["counters"], ref["counters"])
@@ -75,9 +82,10 @@ def test_f64_config1_bit_exact(oracle_lib):
                                atol=1e-12 * np.abs(ref["image"]).max())
 
 
-def test_f64_amr_bit_exact(oracle_lib):
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+def test_f64_amr_bit_exact(oracle_lib, sched):
     w = _amr_workload(8000)
-    got = _run_gpu(w)
+    got = _run_gpu(w, sched=sched)
     ref = oracle_lib.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, w.run, end_state=True)
     _compare_end_states(got["end_state"], ref["end_state"])
     assert np.array_equal(got["counters"], ref["counters"])
@@ -115,3 +123,13 @@ def test_f32_matches_f64_within_noise():
     d = np.abs(m32 - m64)
     # same random numbers: FP32 and FP64 differ far less than the MC noise
     assert np.all(d <= 3.0 * np.sqrt(e64 ** 2 + e32 ** 2) + 1e-9)
+
+
+def test_f32_persistent_equals_history():
+    """Both schedules run the same FP32 per-packet arithmetic: end states agree."""
+    w = _amr_workload(4000, precision=_abi.KP_F32)
+    a = _run_gpu(w, sched=SCHEDULES[0])
+    b = _run_gpu(w, sched=SCHEDULES[1])
+    np.testing.assert_array_equal(a["end_state"], b["end_state"])
+    assert np.array_equal(a["counters"], b["counters"])
+    np.testing.assert_allclose(a["J"], b["J"], rtol=2e-6, atol=1e-9 * np.abs(b["J"]).max())
