@@ -407,14 +407,16 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             const uint64_t avail = wend - wbase;
             uint64_t my = wbase + rank;
             if (n > avail) {
+                // claim enough for every idle lane, at least one chunk
+                const uint64_t take = (n - avail) > P.chunk ? (n - avail) : P.chunk;
                 uint64_t nb = 0;
                 if (lane == __ffsll((long long)need) - 1)
-                    nb = atomicAdd(work, (unsigned long long)P.chunk);
+                    nb = atomicAdd(work, (unsigned long long)take);
                 nb = __shfl(nb, __ffsll((long long)need) - 1, 64);
                 if (rank >= avail)
                     my = nb + (rank - avail);
                 wbase = nb + (n - avail);
-                wend = nb + P.chunk;
+                wend = nb + take;
             } else {
                 wbase += n;
             }
