@@ -207,6 +207,11 @@ typedef struct kp_schedule {
     int32_t block_threads;  /* 64, 128 or 256 (the reference's [particle] n_th) */
     int32_t blocks_per_cu;
     int32_t chunk;
+    /* FP64 atomics on hot cells/pixels are spread over replicas of J and of
+     * the images (workgroup b uses replica b % R), folded into the caller's
+     * arrays after the launch.  0 = choose from a device-memory budget. */
+    int32_t j_replicas;
+    int32_t img_replicas;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

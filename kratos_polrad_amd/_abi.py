@@ -91,6 +91,8 @@ class Schedule(C.Structure):
         ("block_threads", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("chunk", C.c_int32),
+        ("j_replicas", C.c_int32),
+        ("img_replicas", C.c_int32),
     ]
 
 

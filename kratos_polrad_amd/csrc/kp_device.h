@@ -75,7 +75,20 @@ struct kparams {
     // persistent schedule
     unsigned long long *work; // slot counter, zeroed before the launch
     uint32_t chunk;           // slots a wave claims per refill
+    // J and image are replicated R times; workgroup b accumulates into
+    // replica b % R, spreading FP64 atomics on hot cells/pixels
+    uint32_t j_nrep, img_nrep;
+    uint64_t j_stride, img_stride; // elements per replica
 };
+
+__device__ __forceinline__ double *j_base(const kparams &P)
+{
+    return P.J + (uint64_t)(blockIdx.x % P.j_nrep) * P.j_stride;
+}
+__device__ __forceinline__ double *img_base(const kparams &P)
+{
+    return P.image + (uint64_t)(blockIdx.x % P.img_nrep) * P.img_stride;
+}
 
 __device__ __forceinline__ void atomic_add_f64(double *p, double v)
 {

@@ -47,9 +47,12 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0};
     int n_cu = 0;
     unsigned long long *work = nullptr; // persistent slot counter
+    // tally / image replicas (zero between launches)
+    double *j_rep = nullptr, *img_rep = nullptr;
+    size_t j_rep_cap = 0, img_rep_cap = 0; // elements
 };
 
 namespace {
@@ -147,6 +150,27 @@ __global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+// out[e] += sum_r rep[r][e]; rep[r][e] = 0 (ready for the next launch)
+__global__ void __launch_bounds__(256) k_fold(double *out, double *rep, uint64_t n, uint32_t R)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        double s = 0.0;
+        for (uint32_t r = 0; r < R; ++r) {
+            s += rep[r * n + e];
+            rep[r * n + e] = 0.0;
+        }
+        out[e] += s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+        p[e] = 0.0;
+}
+
 template <class Pol> __global__ void __launch_bounds__(256) k_persist(kp::kparams P)
 {
     kp::lane_counters C;
@@ -172,6 +196,9 @@ void free_all(kp_engine *e)
     dfree(e->bands_d);
     dfree(e->band_start_d);
     dfree(e->work);
+    dfree(e->j_rep);
+    dfree(e->img_rep);
+    e->j_rep_cap = e->img_rep_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
     e->stage = nullptr;
@@ -240,6 +267,34 @@ void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t 
                           (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
     KP_HIP(hipStreamSynchronize(stream));
     e->shadow = img;
+}
+
+// Replica count for an array of n elements: the schedule's value, or as many
+// as fit a memory budget (capped by the number of resident workgroups).
+uint32_t pick_replicas(int requested, uint64_t n, uint64_t blocks, uint64_t budget_bytes)
+{
+    uint64_t R = requested > 0 ? (uint64_t)requested : budget_bytes / (n * sizeof(double) + 1);
+    if (R > blocks)
+        R = blocks;
+    if (R > 256)
+        R = 256;
+    if (R < 1)
+        R = 1;
+    return (uint32_t)R;
+}
+
+// Make sure `buf` holds R*n zeroed doubles (fresh allocations are cleared once;
+// afterwards k_fold leaves them zero).
+void ensure_replicas(double *&buf, size_t &cap, uint64_t need, hipStream_t stream)
+{
+    if (cap >= need)
+        return;
+    dfree(buf);
+    cap = 0;
+    buf = dmalloc<double>(need);
+    hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, stream, buf, need);
+    KP_HIP(hipGetLastError());
+    cap = need;
 }
 
 } // namespace
@@ -447,11 +502,41 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         if (total == 0)
             return;
         const kp_schedule &S = e->sched;
+        uint64_t blocks;
+        int T;
         if (S.kind == KP_SCHED_HISTORY) {
-            const int T = 256;
-            uint64_t blocks = (total + T - 1) / T;
+            T = 256;
+            blocks = (total + T - 1) / T;
             if (blocks > 0x7fffffffull)
                 blocks = 0x7fffffffull;
+        } else {
+            T = S.block_threads;
+            blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
+            const uint64_t need = (total + T - 1) / T;
+            if (blocks > need)
+                blocks = need;
+        }
+        // replicas of the FP64 accumulators
+        const uint64_t nJ = (uint64_t)e->n_blocks * e->n3 * run->n_bands;
+        const uint64_t nI = (uint64_t)run->n_bands * run->n_incl * run->image_n * run->image_n * 3;
+        P.j_nrep = P.img_nrep = 1;
+        P.j_stride = nJ;
+        P.img_stride = nI;
+        if (P.J) {
+            P.j_nrep = pick_replicas(S.j_replicas, nJ, blocks, 1ull << 31);
+            if (P.j_nrep > 1) {
+                ensure_replicas(e->j_rep, e->j_rep_cap, (uint64_t)P.j_nrep * nJ, stream);
+                P.J = e->j_rep;
+            }
+        }
+        if (P.image && nI > 0) {
+            P.img_nrep = pick_replicas(S.img_replicas, nI, blocks, 1ull << 29);
+            if (P.img_nrep > 1) {
+                ensure_replicas(e->img_rep, e->img_rep_cap, (uint64_t)P.img_nrep * nI, stream);
+                P.image = e->img_rep;
+            }
+        }
+        if (S.kind == KP_SCHED_HISTORY) {
             if (run->precision == KP_F64)
                 hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)blocks), dim3(T), 0, stream, P);
             else
@@ -460,17 +545,20 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
             P.work = e->work;
             P.chunk = (uint32_t)S.chunk;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
-            uint64_t blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
-            const uint64_t need = (total + S.block_threads - 1) / S.block_threads;
-            if (blocks > need)
-                blocks = need;
             if (run->precision == KP_F64)
-                hipLaunchKernelGGL(k_persist<kp::pol64>, dim3((unsigned)blocks),
-                                   dim3(S.block_threads), 0, stream, P);
+                hipLaunchKernelGGL(k_persist<kp::pol64>, dim3((unsigned)blocks), dim3(T), 0,
+                                   stream, P);
             else
-                hipLaunchKernelGGL(k_persist<kp::pol32>, dim3((unsigned)blocks),
-                                   dim3(S.block_threads), 0, stream, P);
+                hipLaunchKernelGGL(k_persist<kp::pol32>, dim3((unsigned)blocks), dim3(T), 0,
+                                   stream, P);
         }
+        KP_HIP(hipGetLastError());
+        if (P.J && P.j_nrep > 1)
+            hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, stream, out->J, e->j_rep, nJ,
+                               P.j_nrep);
+        if (P.image && P.img_nrep > 1)
+            hipLaunchKernelGGL(k_fold, dim3(512), dim3(256), 0, stream, out->image, e->img_rep,
+                               nI, P.img_nrep);
         KP_HIP(hipGetLastError());
     });
 }
@@ -488,6 +576,9 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: blocks_per_cu must be in [1,64]");
         if (s->chunk < 1 || s->chunk > 4096)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: chunk must be in [1,4096]");
+        if (s->j_replicas < 0 || s->j_replicas > 256 || s->img_replicas < 0 ||
+            s->img_replicas > 256)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: replicas must be in [0,256]");
         e->sched = *s;
     });
 }

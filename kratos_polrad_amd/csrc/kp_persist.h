@@ -464,14 +464,14 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     if (li > l)
                         li = l;
                     if (P.J)
-                        atomic_add_f64(P.J + gc * P.n_bands + L.band, L.I * (li / V));
+                        atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (li / V));
                     Pol::advance(L.cur, li);
                     interact<Pol>(P, L, C, gc);
                     continue;
                 }
                 L.tau = L.tau - dtau;
                 if (P.J)
-                    atomic_add_f64(P.J + gc * P.n_bands + L.band, L.I * (l / V));
+                    atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (l / V));
             } else {
                 const float iV = f32::fast_rcp(P.vol32[L.cur.ib]);
                 if (kap > 0.0f && (double)dtau >= L.tau) {

@@ -214,9 +214,9 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     if (fx < 0.0f || fy < 0.0f || fx >= (float)N || fy >= (float)N)
         return;
     const size_t pix = (((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
-    atomic_add_f64(P.image + 3 * pix + 0, (double)I);
-    atomic_add_f64(P.image + 3 * pix + 1, (double)Q);
-    atomic_add_f64(P.image + 3 * pix + 2, (double)U);
+    atomic_add_f64(img_base(P) + 3 * pix + 0, (double)I);
+    atomic_add_f64(img_base(P) + 3 * pix + 1, (double)Q);
+    atomic_add_f64(img_base(P) + 3 * pix + 2, (double)U);
 }
 
 __device__ __forceinline__ void phase(int model, float gg, float pmax, float mu, float &z11,
@@ -376,7 +376,7 @@ struct tally_acc {
     __device__ __forceinline__ void flush(const kparams &P, int band)
     {
         if (gc >= 0 && P.J)
-            atomic_add_f64(P.J + (size_t)gc * P.n_bands + band, (double)s - (double)c);
+            atomic_add_f64(j_base(P) + (size_t)gc * P.n_bands + band, (double)s - (double)c);
         gc = -1;
     }
     __device__ __forceinline__ void add(const kparams &P, int band, int64_t cell, float v)

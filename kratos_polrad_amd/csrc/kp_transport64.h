@@ -212,9 +212,9 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
         return;
     const size_t pix = (((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
-    atomic_add_f64(P.image + 3 * pix + 0, I);
-    atomic_add_f64(P.image + 3 * pix + 1, Q);
-    atomic_add_f64(P.image + 3 * pix + 2, U);
+    atomic_add_f64(img_base(P) + 3 * pix + 0, I);
+    atomic_add_f64(img_base(P) + 3 * pix + 1, Q);
+    atomic_add_f64(img_base(P) + 3 * pix + 2, U);
 }
 
 __device__ __forceinline__ void phase(int model, double gg, double pmax, double mu, double &z11,
@@ -419,7 +419,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
                 if (li > l)
                     li = l;
                 if (P.J)
-                    atomic_add_f64(P.J + gc * P.n_bands + band, I * (li / V));
+                    atomic_add_f64(j_base(P) + gc * P.n_bands + band, I * (li / V));
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
                     r.x[a] = r.x[a] + li * r.d[a];
@@ -428,7 +428,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
             }
             tau_rem = tau_rem - dtau;
             if (P.J)
-                atomic_add_f64(P.J + gc * P.n_bands + band, I * (l / V));
+                atomic_add_f64(j_base(P) + gc * P.n_bands + band, I * (l / V));
             bool reload;
             const int st = cross_face(P, r, ap, l, reload);
             if (st == ST_ESCAPED) {

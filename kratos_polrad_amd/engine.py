@@ -104,8 +104,9 @@ class Engine:
         return self._h
 
     def set_schedule(self, kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4,
-                     chunk=64) -> None:
-        s = _abi.Schedule(int(kind), int(block_threads), int(blocks_per_cu), int(chunk))
+                     chunk=64, j_replicas=0, img_replicas=0) -> None:
+        s = _abi.Schedule(int(kind), int(block_threads), int(blocks_per_cu), int(chunk),
+                          int(j_replicas), int(img_replicas))
         _abi.check(_abi.lib().kp_set_schedule(self._h, C.byref(s)))
 
     def get_schedule(self) -> dict:
