@@ -1,16 +1,21 @@
 #!/usr/bin/env python3
 """Photon-transport benchmark: cell-crossings/s on one MI355X.
 
-A step is one kp_forward launch over the whole workload (every packet of
-every band: emission, march, scattering, peel-off).  The timed region
-covers exactly --steps launches with inputs resident in HBM, bracketed by
-device synchronisation; kernel time is also taken with HIP events on the
-launch stream.  One JSON line is printed.
+Workload: the 3-level block-AMR configuration (16^3-cell blocks, ~2,200
+blocks, rho ~ r^-2 x lognormal, HG g=0.6, albedo 0.5, Rayleigh-form
+polarization p_max=0.5, point source, 1e8 packets per step, 128x128 I/Q/U
+peel-off image at 90 degrees, J per cell).
 
-The headline value is the mixed-precision path (FP32 block-local transport,
-FP64 optical depth and tallies); the plain FP64 path is timed on the same
-workload, and the Stokes-image error of both is reported against an FP64
-reference run with an independent packet range.
+A step is one kp_forward launch over all packets of the step (emission,
+march, interactions, peel-offs, tallies).  Steps use disjoint packet-id
+ranges.  The timed region covers exactly --steps launches with inputs
+resident in HBM, bracketed by device synchronisation; HIP events on the
+launch stream time the same region.
+
+value = all cell crossings (packet paths + peel-off rays) / wall time of the
+mixed-precision path (FP32 block-local transport, FP64 optical depth,
+FP64 tallies).  The plain FP64 path runs the same steps; both images are
+compared with an independent FP64 run to show equal Stokes-image error.
 """
 import argparse
 import json
@@ -21,7 +26,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12  # bytes/s
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 
 
 def parse():
@@ -29,25 +34,42 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="config1")
-    p.add_argument("--packets", type=float, default=0)
+    p.add_argument("--workload", default="config2", choices=["config1", "config2"])
+    p.add_argument("--packets", type=float, default=0, help="packets per step")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--skip-f64", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run a plain 512 MiB device-to-device copy (3 times) so "
+                        "rocprofv3 counters can be scaled by a kernel of known traffic")
     return p.parse_args()
 
 
 def algorithmic_bytes(cnt, n_incl):
-    """B_alg = 20 B per crossing (4 B kappa + 16 B FP64 tally read-modify-write;
-    peel-off crossings read kappa only: 4 B) + 48 B per peel-off pixel update."""
+    """B_alg per the hot-path scope: 20 B per packet-path crossing (4 B kappa_rho
+    read + 16 B FP64 J read-modify-write), 4 B per peel-off crossing (kappa_rho
+    only), 48 B per peel-off pixel update (FP64 I,Q,U read-modify-write).
+    Packets live in registers for their whole history: no packet-pool traffic."""
     main, peel, events, packets = cnt[0], cnt[1], cnt[2], cnt[7]
     peel_px = (events + packets) * n_incl
     return 20.0 * main + 4.0 * peel + 48.0 * peel_px
 
 
+def img_err(a, b):
+    """relative L2 difference of the I, Q, U images (a vs reference b)."""
+    import numpy as np
+
+    out = {}
+    for k, name in enumerate("IQU"):
+        x, y = a[..., k].ravel(), b[..., k].ravel()
+        out[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(b[..., 0].ravel()), 1e-300))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU; run with --gpus 1"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -55,11 +77,22 @@ def main():
     from kratos_polrad_amd import _abi, configs
     from kratos_polrad_amd.engine import Engine, RunSpec
 
+    if args.copy_calibration:
+        nbytes = 512 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": nbytes}), flush=True)
+        del src, dst
+
     if args.workload == "config1":
         n = int(args.packets) or 1_000_000
         w = configs.config1(n_packets=n, precision=_abi.KP_F32)
     else:
-        raise SystemExit("unknown workload")
+        n = int(args.packets) or 100_000_000
+        w = configs.config2(n_packets=n, precision=_abi.KP_F32)
     eng = Engine(w.mesh)
     eng.set_media(w.kappa, w.albedo, w.g)
     stream = torch.cuda.current_stream()
@@ -69,60 +102,32 @@ def main():
         run = RunSpec(**{**w.run.__dict__, "precision": prec})
         out = eng.alloc_outputs(run)
         for k in range(warmup):
-            run.packet_offset = offset0 + 10**12 + k * run.total_packets
+            run.packet_offset = (1 << 50) + k * n
             eng.forward(w.source, run, out)
         torch.cuda.synchronize()
         for v in out.values():
             v.zero_()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record(stream)
         for k in range(steps):
-            run.packet_offset = offset0 + k * run.total_packets
+            run.packet_offset = offset0 + k * n
             eng.forward(w.source, run, out)
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kern = ev0.elapsed_time(ev1) / 1e3
         cnt = out["counters"].cpu().numpy().astype(np.float64)
-        img = out["image"].cpu().numpy() if "image" in out else None
+        img = out["image"].cpu().numpy()[0, 0]
         return wall, kern, cnt, img
 
     wall32, kern32, cnt32, img32 = timed(_abi.KP_F32, args.steps, args.warmup)
-    wall64, kern64, cnt64, img64 = timed(_abi.KP_F64, args.steps, args.warmup)
     cross32 = cnt32[0] + cnt32[1]
-    cross64 = cnt64[0] + cnt64[1]
     value = cross32 / wall32
     b_alg = algorithmic_bytes(cnt32, n_incl) / args.steps
     achieved = b_alg / (kern32 / args.steps)
-
-    # Stokes-image error of each path against an independent FP64 run
-    def img_err(img, ref):
-        a = img[..., 0].sum(axis=(0, 1))
-        b = ref[..., 0].sum(axis=(0, 1))
-        return float(np.abs(a - b).sum() / np.abs(b).sum())
-
-    _, _, _, ref = timed(_abi.KP_F64, args.steps, 0, offset0=10**13)
-    err32 = img_err(img32, ref)
-    err64 = img_err(img64, ref)
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-
-        n_cpu = max(1000, int(2e5 * args.cpu_seconds / 15.0))
-        run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
-        run.bands = [type(run.bands[0])(**{**run.bands[0].__dict__, "n_packets": n_cpu})]
-        cores = len(os.sched_getaffinity(0))
-        cores = min(cores, 16)
-        t0 = time.perf_counter()
-        o = oracle.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, run, n_threads=cores)
-        dt = time.perf_counter() - t0
-        c = o["counters"].astype(np.float64)
-        cpu = {"value": float((c[0] + c[1]) / dt), "unit": "cell-crossings/s", "cores": cores,
-               "kind": "port",
-               "sample": f"{n_cpu} packets of the same workload, FP64 oracle, {dt:.1f} s"}
 
     res = {
         "metric": "cell-crossings/s at fixed Stokes-image error vs FP64",
@@ -135,20 +140,61 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 transport / f64 tau+tallies",
+        "dtype": "f32 transport, f64 optical depth and tallies",
         "data": "synthetic",
-        "config": {"workload": w.name, "packets": w.run.total_packets,
-                   "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
-                   "n_incl": n_incl, "image_n": w.run.image_n},
-        "fp64_path": {"value": float(cross64 / wall64), "ms_per_step": 1e3 * wall64 / args.steps,
-                      "image_err_vs_fp64": err64},
-        "mixed_path_image_err_vs_fp64": err32,
-        "speedup_mixed_vs_fp64": float((cross32 / wall32) / (cross64 / wall64)),
-        "crossings_per_step": {"main": cnt32[0] / args.steps, "peel": cnt32[1] / args.steps},
+        "config": {"workload": w.name, "packets_per_step": n, "blocks": w.mesh.n_blocks,
+                   "cells": w.mesh.n_cells, "levels": int(w.mesh.level.max()) + 1,
+                   "n_incl": n_incl, "image_n": w.run.image_n,
+                   "schedule": eng.get_schedule()},
+        "crossings_per_step": {"main": cnt32[0] / args.steps, "peel": cnt32[1] / args.steps,
+                               "events": cnt32[2] / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
-        "cpu_baseline": cpu,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_step": b_alg,
+                     "kernel_ms_per_step": 1e3 * kern32 / args.steps},
     }
+
+    if not args.skip_f64:
+        wall64, kern64, cnt64, img64 = timed(_abi.KP_F64, args.steps, 0)
+        # independent FP64 reference (other packet ids) for the image error
+        _, _, _, ref = timed(_abi.KP_F64, args.steps, 0, offset0=1 << 40)
+        value64 = (cnt64[0] + cnt64[1]) / wall64
+        res["fp64_path"] = {"value": float(value64), "ms_per_step": 1e3 * wall64 / args.steps}
+        res["speedup_mixed_vs_fp64"] = float(value / value64)
+        res["image_error"] = {
+            "mixed_vs_fp64_same_packets": img_err(img32, img64),
+            "mixed_vs_independent_fp64": img_err(img32, ref),
+            "fp64_vs_independent_fp64": img_err(img64, ref),
+        }
+
+    if not args.no_cpu:
+        import oracle
+
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+        b0 = run.bands[0]
+
+        def cpu_run(npk):
+            run.bands = [type(b0)(**{**b0.__dict__, "n_packets": npk})]
+            t0 = time.perf_counter()
+            o = oracle.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, run, n_threads=cores)
+            return time.perf_counter() - t0, o["counters"].astype(np.float64)
+
+        # pilots to size a sample of about --cpu-seconds of CPU work
+        npk = 2000
+        dt, c = cpu_run(npk)
+        while dt < 1.0 and npk < 5e7:
+            npk *= 8
+            dt, c = cpu_run(npk)
+        if dt < 0.5 * args.cpu_seconds:
+            npk = int(min(npk * args.cpu_seconds / max(dt, 1e-3), 5e7))
+            dt, c = cpu_run(npk)
+        n_cpu = npk
+        res["cpu_baseline"] = {
+            "value": float((c[0] + c[1]) / dt), "unit": "cell-crossings/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{n_cpu} packets of the same workload (FP64 oracle, pthreads), "
+                      f"{dt:.1f} s"}
     print(json.dumps(res))
 
 
