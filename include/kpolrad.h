@@ -212,6 +212,10 @@ typedef struct kp_schedule {
      * arrays after the launch.  0 = choose from a device-memory budget. */
     int32_t j_replicas;
     int32_t img_replicas;
+    /* persistent kernel: a wave runs its pending interactions / peel-off
+     * ends / packet ends together once this many lanes wait (1..64) */
+    int32_t event_batch;
+    int32_t pad_;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

@@ -93,6 +93,8 @@ class Schedule(C.Structure):
         ("chunk", C.c_int32),
         ("j_replicas", C.c_int32),
         ("img_replicas", C.c_int32),
+        ("event_batch", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 

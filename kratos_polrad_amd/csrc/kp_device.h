@@ -75,6 +75,7 @@ struct kparams {
     // persistent schedule
     unsigned long long *work; // slot counter, zeroed before the launch
     uint32_t chunk;           // slots a wave claims per refill
+    uint32_t batch;           // waiting lanes that trigger an action phase
     // J and image are replicated R times; workgroup b accumulates into
     // replica b % R, spreading FP64 atomics on hot cells/pixels
     uint32_t j_nrep, img_nrep;

@@ -47,7 +47,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0};
     int n_cu = 0;
     unsigned long long *work = nullptr; // persistent slot counter
     // tally / image replicas (zero between launches)
@@ -270,7 +270,9 @@ void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t 
 }
 
 // Replica count for an array of n elements: the schedule's value, or as many
-// as fit a memory budget (capped by the number of resident workgroups).
+// as fit a byte budget (capped by the number of resident workgroups).  The
+// default budgets keep the replicas inside the 256 MB Infinity Cache: replicas
+// that spill to HBM turn every FP64 atomic into a line fetch + write-back.
 uint32_t pick_replicas(int requested, uint64_t n, uint64_t blocks, uint64_t budget_bytes)
 {
     uint64_t R = requested > 0 ? (uint64_t)requested : budget_bytes / (n * sizeof(double) + 1);
@@ -523,14 +525,14 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         P.j_stride = nJ;
         P.img_stride = nI;
         if (P.J) {
-            P.j_nrep = pick_replicas(S.j_replicas, nJ, blocks, 1ull << 31);
+            P.j_nrep = pick_replicas(S.j_replicas, nJ, blocks, 160ull << 20);
             if (P.j_nrep > 1) {
                 ensure_replicas(e->j_rep, e->j_rep_cap, (uint64_t)P.j_nrep * nJ, stream);
                 P.J = e->j_rep;
             }
         }
         if (P.image && nI > 0) {
-            P.img_nrep = pick_replicas(S.img_replicas, nI, blocks, 1ull << 29);
+            P.img_nrep = pick_replicas(S.img_replicas, nI, blocks, 32ull << 20);
             if (P.img_nrep > 1) {
                 ensure_replicas(e->img_rep, e->img_rep_cap, (uint64_t)P.img_nrep * nI, stream);
                 P.image = e->img_rep;
@@ -544,6 +546,7 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         } else {
             P.work = e->work;
             P.chunk = (uint32_t)S.chunk;
+            P.batch = (uint32_t)S.event_batch;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
             if (run->precision == KP_F64)
                 hipLaunchKernelGGL(k_persist<kp::pol64>, dim3((unsigned)blocks), dim3(T), 0,
@@ -576,6 +579,8 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: blocks_per_cu must be in [1,64]");
         if (s->chunk < 1 || s->chunk > 4096)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: chunk must be in [1,4096]");
+        if (s->event_batch < 1 || s->event_batch > 64)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: event_batch must be in [1,64]");
         if (s->j_replicas < 0 || s->j_replicas > 256 || s->img_replicas < 0 ||
             s->img_replicas > 256)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: replicas must be in [0,256]");

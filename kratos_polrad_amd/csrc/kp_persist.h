@@ -3,18 +3,22 @@
 // The reference runs each packet's whole history in one thread of one
 // launch (integrator.h:80-106 -> photon.h:104-125).  Here every lane runs a
 // small state machine instead:
-//   NEED  - take the next packet slot from the wave's slot range (the wave
-//           claims CHUNK slots per global atomic, handed to lanes by ballot
-//           rank), emit it;
 //   MAIN  - one crossing of the packet's own path (tally, optical depth);
 //   PEEL  - one crossing of a peel-off ray toward an observer;
+//   PEND  - waiting for the wave's next action phase with a pending action:
+//           an interaction, the end of a peel-off ray, or the end of the
+//           packet;
+//   NEED  - waiting for a new packet (the wave claims at least `chunk` slots
+//           per global atomic and hands them out by ballot rank);
 //   DONE  - no work left.
 // Main-path and peel-off crossings execute the same face-distance and
-// block-change code, so a wave stays converged on the crossing loop while
-// its lanes are at different points of different histories; emission and
-// interactions are the only divergent branches.  The arithmetic of each
-// history is exactly that of the per-packet functions of the precision
-// policy (bit-identical to the CPU oracle in FP64).
+// block-change code, so the wave stays converged on the crossing loop.  The
+// long, divergent code (Philox, scattering, Stokes rotation, peel-off set-up,
+// deposits, emission) runs in action phases, entered when at least
+// `event_batch` lanes (or half the live lanes, or all marching lanes) are
+// waiting, so it executes once for many lanes instead of once per lane.
+// Each history performs exactly the per-packet arithmetic of the precision
+// policy in the same order (bit-identical to the CPU oracle in FP64).
 #pragma once
 
 #include "kp_transport32.h"
@@ -22,7 +26,8 @@
 
 namespace kp {
 
-enum : int { M_NEED = 0, M_MAIN = 1, M_PEEL = 2, M_DONE = 3 };
+enum : int { M_NEED = 0, M_MAIN = 1, M_PEEL = 2, M_DONE = 3, M_PEND = 4 };
+enum : int { PA_NONE = 0, PA_INTERACT = 1, PA_PEEL_END = 2, PA_SCATTER = 3, PA_FINISH = 4 };
 enum : int { EV_EMIT = 0, EV_SCATTER = 1 };
 
 // ---------------------------------------------------------------- policies
@@ -137,6 +142,8 @@ template <class Pol> struct lane_t {
     uint64_t slot, pid;
     int band;
     int mode, ev_kind, o;
+    int pa, status;   // pending action and, for PA_FINISH, the end status
+    int64_t gc;       // cell of the pending interaction
     int events, steps;
     uint32_t ncross, npeel;
     f32::tally_acc acc; // FP32: Kahan register accumulator of the current cell
@@ -235,12 +242,13 @@ template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
     L.tau_main = L.tau;
 }
 
-// After the interaction's peel-offs: scatter, new free flight, roulette.
-template <class Pol>
-__device__ __forceinline__ void finish_event(const kparams &P, lane_t<Pol> &L, lane_counters &C)
+// Scatter, new free flight, Russian roulette (after the interaction's
+// peel-offs).  Leaves the lane in MAIN, or pending PA_FINISH when killed.
+template <class Pol> __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
     real e2[3];
+    bool killed = false;
     if constexpr (sizeof(real) == 8) {
         f64::cross3(L.cur.d, L.e1, e2);
         const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(L.u[0]))
@@ -254,8 +262,7 @@ __device__ __forceinline__ void finish_event(const kparams &P, lane_t<Pol> &L, l
                 L.Q = L.Q / P.rr_s;
                 L.U = L.U / P.rr_s;
             } else {
-                finish(P, L, C, KP_END_RR);
-                return;
+                killed = true;
             }
         }
     } else {
@@ -273,19 +280,24 @@ __device__ __forceinline__ void finish_event(const kparams &P, lane_t<Pol> &L, l
                 L.Q *= inv;
                 L.U *= inv;
             } else {
-                finish(P, L, C, KP_END_RR);
-                return;
+                killed = true;
             }
         }
     }
-    L.mode = M_MAIN;
+    if (killed) {
+        L.pa = PA_FINISH;
+        L.status = KP_END_RR;
+    } else {
+        L.mode = M_MAIN;
+        L.pa = PA_NONE;
+    }
 }
 
-template <class Pol>
-__device__ __forceinline__ void end_peel(const kparams &P, lane_t<Pol> &L, lane_counters &C)
+// End of a peel-off ray: deposit, then the next observer, or resume the
+// main path (EMIT: march on; SCATTER: pending scatter).
+template <class Pol> __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
-    // deposit at the projected interaction (or emission) point
     if (P.image) {
         if constexpr (sizeof(real) == 8) {
             const double att = kp_exp(-L.tau) * KP_INV_4PI;
@@ -305,9 +317,9 @@ __device__ __forceinline__ void end_peel(const kparams &P, lane_t<Pol> &L, lane_
     L.o += 1;
     if (L.o < P.n_incl) {
         start_peel<Pol>(P, L);
+        L.pa = PA_NONE;
         return;
     }
-    // resume the main path
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         L.cur.x[a] = L.sx[a];
@@ -317,10 +329,12 @@ __device__ __forceinline__ void end_peel(const kparams &P, lane_t<Pol> &L, lane_
     L.cur.ib = L.sib;
     Pol::relocate(P, L.cur);
     L.tau = L.tau_main;
-    if (L.ev_kind == EV_EMIT)
+    if (L.ev_kind == EV_EMIT) {
         L.mode = M_MAIN;
-    else
-        finish_event<Pol>(P, L, C);
+        L.pa = PA_NONE;
+    } else {
+        L.pa = PA_SCATTER;
+    }
 }
 
 template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
@@ -341,6 +355,7 @@ template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &
     L.steps = P.step_max;
     L.ncross = 0;
     L.npeel = 0;
+    L.pa = PA_NONE;
     if (P.n_incl > 0) {
         save_main(L);
         L.ev_kind = EV_EMIT;
@@ -351,11 +366,13 @@ template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &
     }
 }
 
+// Interaction at the current point: Philox words of the event, absorption,
+// then the peel-offs (or the scatter directly when there is no observer).
 template <class Pol>
-__device__ __forceinline__ void interact(const kparams &P, lane_t<Pol> &L, lane_counters &C,
-                                         int64_t gc)
+__device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
+    const int64_t gc = L.gc;
     L.events += 1;
     L.u[0] = (uint32_t)L.pid;
     L.u[1] = (uint32_t)(L.pid >> 32);
@@ -383,8 +400,9 @@ __device__ __forceinline__ void interact(const kparams &P, lane_t<Pol> &L, lane_
         L.ev_kind = EV_SCATTER;
         L.o = 0;
         start_peel<Pol>(P, L);
+        L.pa = PA_NONE;
     } else {
-        finish_event<Pol>(P, L, C);
+        L.pa = PA_SCATTER;
     }
 }
 
@@ -394,60 +412,76 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     using real = typename Pol::real;
     lane_t<Pol> L;
     L.mode = M_NEED;
+    L.pa = PA_NONE;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
 
     for (;;) {
-        // ---- refill idle lanes from the wave's slot range
-        const uint64_t need = __ballot(L.mode == M_NEED);
-        if (need) {
-            const uint32_t n = __popcll(need);
-            const uint32_t rank = __popcll(need & lt_mask);
-            const uint64_t avail = wend - wbase;
-            uint64_t my = wbase + rank;
-            if (n > avail) {
-                // claim enough for every idle lane, at least one chunk
-                const uint64_t take = (n - avail) > P.chunk ? (n - avail) : P.chunk;
-                uint64_t nb = 0;
-                if (lane == __ffsll((long long)need) - 1)
-                    nb = atomicAdd(work, (unsigned long long)take);
-                nb = __shfl(nb, __ffsll((long long)need) - 1, 64);
-                if (rank >= avail)
-                    my = nb + (rank - avail);
-                wbase = nb + (n - avail);
-                wend = nb + take;
-            } else {
-                wbase += n;
-            }
-            if (L.mode == M_NEED) {
-                if (my < P.total) {
-                    L.slot = my;
-                    emit_packet<Pol>(P, L);
+        const uint64_t live = __ballot(L.mode != M_DONE);
+        if (!live)
+            break;
+        const uint64_t wait = __ballot(L.mode == M_PEND || L.mode == M_NEED);
+        const uint64_t march = __ballot(L.mode == M_MAIN || L.mode == M_PEEL);
+        const uint32_t nw = __popcll(wait);
+        if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
+            // ---------------- action phase: every pending lane together
+            if (L.mode == M_PEND && L.pa == PA_PEEL_END)
+                peel_end_stage<Pol>(P, L);
+            if (L.mode == M_PEND && L.pa == PA_INTERACT)
+                interact_stage<Pol>(P, L);
+            if (L.mode == M_PEND && L.pa == PA_SCATTER)
+                scatter_stage<Pol>(P, L);
+            if (L.mode == M_PEND && L.pa == PA_FINISH)
+                finish(P, L, C, L.status);
+            // refill idle lanes from the wave's slot range
+            const uint64_t need = __ballot(L.mode == M_NEED);
+            if (need) {
+                const uint32_t n = __popcll(need);
+                const uint32_t rank = __popcll(need & lt_mask);
+                const uint64_t avail = wend - wbase;
+                uint64_t my = wbase + rank;
+                if (n > avail) {
+                    // claim enough for every idle lane, at least one chunk
+                    const uint64_t take = (n - avail) > P.chunk ? (n - avail) : P.chunk;
+                    uint64_t nb = 0;
+                    if (lane == __ffsll((long long)need) - 1)
+                        nb = atomicAdd(work, (unsigned long long)take);
+                    nb = __shfl(nb, __ffsll((long long)need) - 1, 64);
+                    if (rank >= avail)
+                        my = nb + (rank - avail);
+                    wbase = nb + (n - avail);
+                    wend = nb + take;
                 } else {
-                    L.mode = M_DONE;
+                    wbase += n;
+                }
+                if (L.mode == M_NEED) {
+                    if (my < P.total) {
+                        L.slot = my;
+                        emit_packet<Pol>(P, L);
+                    } else {
+                        L.mode = M_DONE;
+                    }
                 }
             }
         }
-        if (!__ballot(L.mode != M_DONE))
-            break;
         if (L.mode != M_MAIN && L.mode != M_PEEL)
             continue;
 
-        // ---- one march step of the current ray
+        // ---------------- one march step of the current ray
         const bool is_main = (L.mode == M_MAIN);
         if (is_main && L.steps <= 0) {
-            finish(P, L, C, KP_END_STEPS);
+            L.mode = M_PEND;
+            L.pa = PA_FINISH;
+            L.status = KP_END_STEPS;
             continue;
         }
         real l;
         const int ap = Pol::dl(L.cur, l);
         if (ap < 0) {
-            if (is_main) {
-                finish(P, L, C, KP_END_LOST);
-            } else {
-                end_peel<Pol>(P, L, C);
-            }
+            L.mode = M_PEND;
+            L.pa = is_main ? PA_FINISH : PA_PEEL_END;
+            L.status = KP_END_LOST;
             continue;
         }
         const int64_t gc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
@@ -466,7 +500,9 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     if (P.J)
                         atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (li / V));
                     Pol::advance(L.cur, li);
-                    interact<Pol>(P, L, C, gc);
+                    L.gc = gc;
+                    L.mode = M_PEND;
+                    L.pa = PA_INTERACT;
                     continue;
                 }
                 L.tau = L.tau - dtau;
@@ -478,7 +514,9 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     const float li = fminf((float)L.tau * f32::fast_rcp(kap), l);
                     L.acc.add(P, L.band, gc, L.I * li * iV);
                     Pol::advance(L.cur, li);
-                    interact<Pol>(P, L, C, gc);
+                    L.gc = gc;
+                    L.mode = M_PEND;
+                    L.pa = PA_INTERACT;
                     continue;
                 }
                 L.tau -= (double)dtau;
@@ -488,16 +526,19 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             L.tau = L.tau + (double)dtau;
             L.npeel += 1;
             if (L.tau > kTauCut) {
-                end_peel<Pol>(P, L, C);
+                L.mode = M_PEND;
+                L.pa = PA_PEEL_END;
                 continue;
             }
         }
         const int st = Pol::cross(P, L.cur, ap, l);
         if (st != f64::ST_IN) {
+            L.mode = M_PEND;
             if (is_main) {
-                finish(P, L, C, st == f64::ST_ESCAPED ? KP_END_ESCAPED : KP_END_LOST);
+                L.pa = PA_FINISH;
+                L.status = (st == f64::ST_ESCAPED) ? KP_END_ESCAPED : KP_END_LOST;
             } else {
-                end_peel<Pol>(P, L, C);
+                L.pa = PA_PEEL_END;
             }
         }
     }

@@ -135,13 +135,14 @@ def test_f32_persistent_equals_history():
     np.testing.assert_allclose(a["J"], b["J"], rtol=2e-6, atol=1e-9 * np.abs(b["J"]).max())
 
 
-@pytest.mark.parametrize("chunk,bt", [(1, 64), (16, 256), (1000, 128)])
-def test_persistent_refill_processes_each_packet_once(chunk, bt):
-    """Wave slot ranges never overlap: any chunk size gives every packet
-    exactly once, with the same end states as the one-thread-per-packet run."""
+@pytest.mark.parametrize("chunk,bt,batch", [(1, 64, 1), (16, 256, 16), (1000, 128, 64)])
+def test_persistent_refill_processes_each_packet_once(chunk, bt, batch):
+    """Wave slot ranges never overlap and batched action phases keep every
+    history intact: any chunk / batch size gives every packet exactly once,
+    with the same end states as the one-thread-per-packet run."""
     w = _amr_workload(3000, precision=_abi.KP_F32)
     ref = _run_gpu(w, sched=SCHEDULES[1])
     got = _run_gpu(w, sched=dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=bt,
-                                 blocks_per_cu=2, chunk=chunk))
+                                 blocks_per_cu=2, chunk=chunk, event_batch=batch))
     assert int(got["counters"][_abi.KP_C_PACKETS]) == 3000
     np.testing.assert_array_equal(got["end_state"], ref["end_state"])
