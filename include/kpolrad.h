@@ -215,7 +215,9 @@ typedef struct kp_schedule {
     /* persistent kernel: a wave runs its pending interactions / peel-off
      * ends / packet ends together once this many lanes wait (1..64) */
     int32_t event_batch;
-    int32_t pad_;
+    /* register budget of the persistent kernel: minimum waves per SIMD the
+     * compiler must fit (0 = its own choice; FP32: 0,5,6; FP64: 0,3,4) */
+    int32_t waves_per_simd;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

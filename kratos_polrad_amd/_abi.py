@@ -94,7 +94,7 @@ class Schedule(C.Structure):
         ("j_replicas", C.c_int32),
         ("img_replicas", C.c_int32),
         ("event_batch", C.c_int32),
-        ("pad_", C.c_int32),
+        ("waves_per_simd", C.c_int32),
     ]
 
 

@@ -47,7 +47,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0};  // last: waves_per_simd
     int n_cu = 0;
     unsigned long long *work = nullptr; // persistent slot counter
     // tally / image replicas (zero between launches)
@@ -171,11 +171,30 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
         p[e] = 0.0;
 }
 
-template <class Pol> __global__ void __launch_bounds__(256) k_persist(kp::kparams P)
+template <class Pol, int WPS>
+__global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
     kp::lane_counters C;
     kp::persist_body<Pol>(P, P.work, C);
     flush_counters(P, C);
+}
+
+// register-budget variants of the persistent kernel
+using persist_fn = void (*)(kp::kparams);
+persist_fn persist_variant(int precision, int wps)
+{
+    if (precision == KP_F64) {
+        if (wps == 3)
+            return k_persist<kp::pol64, 3>;
+        if (wps == 4)
+            return k_persist<kp::pol64, 4>;
+        return k_persist<kp::pol64, 1>;
+    }
+    if (wps == 5)
+        return k_persist<kp::pol32, 5>;
+    if (wps == 6)
+        return k_persist<kp::pol32, 6>;
+    return k_persist<kp::pol32, 1>;
 }
 
 void free_all(kp_engine *e)
@@ -548,12 +567,8 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
             P.chunk = (uint32_t)S.chunk;
             P.batch = (uint32_t)S.event_batch;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
-            if (run->precision == KP_F64)
-                hipLaunchKernelGGL(k_persist<kp::pol64>, dim3((unsigned)blocks), dim3(T), 0,
-                                   stream, P);
-            else
-                hipLaunchKernelGGL(k_persist<kp::pol32>, dim3((unsigned)blocks), dim3(T), 0,
-                                   stream, P);
+            hipLaunchKernelGGL(persist_variant(run->precision, S.waves_per_simd),
+                               dim3((unsigned)blocks), dim3(T), 0, stream, P);
         }
         KP_HIP(hipGetLastError());
         if (P.J && P.j_nrep > 1)
@@ -579,6 +594,9 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: blocks_per_cu must be in [1,64]");
         if (s->chunk < 1 || s->chunk > 4096)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: chunk must be in [1,4096]");
+        if (s->waves_per_simd != 0 && s->waves_per_simd != 3 && s->waves_per_simd != 4 &&
+            s->waves_per_simd != 5 && s->waves_per_simd != 6)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: waves_per_simd must be 0, 3, 4, 5 or 6");
         if (s->event_batch < 1 || s->event_batch > 64)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: event_batch must be in [1,64]");
         if (s->j_replicas < 0 || s->j_replicas > 256 || s->img_replicas < 0 ||

@@ -144,6 +144,8 @@ template <class Pol> struct lane_t {
     int mode, ev_kind, o;
     int pa, status;   // pending action and, for PA_FINISH, the end status
     int64_t gc;       // cell of the pending interaction
+    int64_t kgc;      // cell whose kappa_rho is in kcache (issued one step ahead)
+    float kcache;
     int events, steps;
     uint32_t ncross, npeel;
     f32::tally_acc acc; // FP32: Kahan register accumulator of the current cell
@@ -413,6 +415,8 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     lane_t<Pol> L;
     L.mode = M_NEED;
     L.pa = PA_NONE;
+    L.kgc = -1;
+    L.kcache = 0.0f;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
@@ -486,7 +490,8 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         }
         const int64_t gc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
         const real ks = (real)P.bands[L.band].kappa_scale;
-        const real kap = (real)P.kappa[gc] * ks;
+        const float kraw = (gc == L.kgc) ? L.kcache : P.kappa[gc];
+        const real kap = (real)kraw * ks;
         const real dtau = kap * l;
         if (is_main) {
             L.ncross += 1;
@@ -532,7 +537,11 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             }
         }
         const int st = Pol::cross(P, L.cur, ap, l);
-        if (st != f64::ST_IN) {
+        if (st == f64::ST_IN) {
+            // issue the next cell's kappa_rho load now; it is consumed next step
+            L.kgc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
+            L.kcache = P.kappa[L.kgc];
+        } else {
             L.mode = M_PEND;
             if (is_main) {
                 L.pa = PA_FINISH;

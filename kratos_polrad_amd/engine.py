@@ -104,15 +104,17 @@ class Engine:
         return self._h
 
     def set_schedule(self, kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4,
-                     chunk=64, j_replicas=0, img_replicas=0, event_batch=8) -> None:
+                     chunk=64, j_replicas=0, img_replicas=0, event_batch=8,
+                     waves_per_simd=0) -> None:
         s = _abi.Schedule(int(kind), int(block_threads), int(blocks_per_cu), int(chunk),
-                          int(j_replicas), int(img_replicas), int(event_batch), 0)
+                          int(j_replicas), int(img_replicas), int(event_batch),
+                          int(waves_per_simd))
         _abi.check(_abi.lib().kp_set_schedule(self._h, C.byref(s)))
 
     def get_schedule(self) -> dict:
         s = _abi.Schedule()
         _abi.check(_abi.lib().kp_get_schedule(self._h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in _abi.Schedule._fields_ if k != "pad_"}
+        return {k: getattr(s, k) for k, _ in _abi.Schedule._fields_}
 
     def set_media(self, kappa_rho, albedo, g=None) -> None:
         n = self.mesh.n_cells
