@@ -228,6 +228,24 @@ kp_status kp_get_schedule(const kp_engine *eng, kp_schedule *sched);
 kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,
                      kp_tallies *out, void *stream);
 
+/* Device gradient outputs of kp_adjoint, per cell (each may be NULL);
+ * kp_adjoint ADDS into them.  jlin (device scalar) receives the primal
+ * value sum(dL_dimage * image) of the replayed histories. */
+typedef struct kp_grads {
+    double *d_logrho;  /* d/d log(rho_c): kappa_rho of cell c scales with rho_c */
+    double *d_albedo;  /* d/d albedo_c (bands with KP_BAND_CELL_ALBEDO)       */
+    double *d_g;       /* d/d g_c      (KP_HG_RAYLEIGH, KP_BAND_CELL_G)        */
+    double *jlin;
+} kp_grads;
+
+/* Gradient of Jlin = sum_pix dL_dimage[pix] . image[pix] (device array in the
+ * kp_tallies image layout) by path replay: every history of the run is
+ * replayed twice from its Philox key, FP64 transport and accumulation.
+ * Same packets as kp_forward with the same run.  No reference counterpart
+ * (the reference has no adjoint); it serves image-fitting callers. */
+kp_status kp_adjoint(kp_engine *eng, const kp_source *src, const kp_run *run,
+                     const double *dL_dimage, kp_grads *grads, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,15 @@ class Tallies(C.Structure):
     ]
 
 
+class Grads(C.Structure):
+    _fields_ = [
+        ("d_logrho", C.c_void_p),
+        ("d_albedo", C.c_void_p),
+        ("d_g", C.c_void_p),
+        ("jlin", C.c_void_p),
+    ]
+
+
 # Every entry point declared in include/kpolrad.h, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -125,6 +134,7 @@ SIGNATURES = {
     "kp_engine_destroy": (None, [_P]),
     "kp_set_media": (C.c_int, [_P, _P, _P, _P]),
     "kp_forward": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.POINTER(Tallies), _P]),
+    "kp_adjoint": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), _P, C.POINTER(Grads), _P]),
     "kp_set_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
     "kp_get_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
 }

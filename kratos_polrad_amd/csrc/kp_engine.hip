@@ -16,6 +16,7 @@
 
 #include "kp_device.h"
 #include "kp_error.h"
+#include "kp_adjoint.h"
 #include "kp_persist.h"
 
 #define KP_HIP(call)                                                                        \
@@ -53,6 +54,9 @@ struct kp_engine {
     // tally / image replicas (zero between launches)
     double *j_rep = nullptr, *img_rep = nullptr;
     size_t j_rep_cap = 0, img_rep_cap = 0; // elements
+    // adjoint: per-thread rows of peel-off contributions
+    double *adj_scratch = nullptr;
+    size_t adj_cap = 0;
 };
 
 namespace {
@@ -197,6 +201,22 @@ persist_fn persist_variant(int precision, int wps)
     return k_persist<kp::pol32, 1>;
 }
 
+__global__ void __launch_bounds__(256) k_adjoint(kp::kparams P, kp::adj::aparams Q)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double jsum = 0.0;
+    for (uint64_t slot = tid; slot < P.total; slot += Q.n_threads) {
+        const int band = kp::band_of(P, slot);
+        const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
+        kp::adj::adjoint_packet(P, Q, band, pid, tid, jsum);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        jsum += __shfl_xor(jsum, off, 64);
+    if ((threadIdx.x & 63) == 0 && Q.jlin && jsum != 0.0)
+        atomicAdd(Q.jlin, jsum);
+}
+
 void free_all(kp_engine *e)
 {
     dfree(e->xf0);
@@ -217,6 +237,8 @@ void free_all(kp_engine *e)
     dfree(e->work);
     dfree(e->j_rep);
     dfree(e->img_rep);
+    dfree(e->adj_scratch);
+    e->adj_cap = 0;
     e->j_rep_cap = e->img_rep_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
@@ -316,6 +338,73 @@ void ensure_replicas(double *&buf, size_t &cap, uint64_t need, hipStream_t strea
     hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, stream, buf, need);
     KP_HIP(hipGetLastError());
     cap = need;
+}
+
+void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStream_t stream,
+                 kp::kparams &P, uint64_t &total)
+{
+        std::memset(&P, 0, sizeof(P));
+        P.xf0 = e->xf0;
+        P.dx0 = e->dx0;
+        P.vol0 = e->vol0;
+        P.nb_id = e->nb_id;
+        P.nb_shift = e->nb_shift;
+        P.d_lvl = e->d_lvl;
+        P.phybnd = e->phybnd;
+        P.h32 = e->h32;
+        P.org32 = e->org32;
+        P.vol32 = e->vol32;
+        P.n_cell = e->n_cell;
+        P.n3 = e->n3;
+        for (int a = 0; a < 3; ++a) {
+            P.periodic[a] = e->periodic[a];
+            P.xmin[a] = e->xmin[a];
+            P.xmax[a] = e->xmax[a];
+            P.center[a] = (e->xmin[a] + e->xmax[a]) * 0.5;
+        }
+        P.kappa = e->kappa;
+        P.albedo = e->albedo;
+        P.g = e->g;
+        total = 0;
+        upload_bands(e, run, total, stream);
+        P.bands = e->bands_d;
+        P.band_start = e->band_start_d;
+        P.n_bands = run->n_bands;
+        P.total = total;
+        P.packet_offset = run->packet_offset;
+        int ib, ii[3];
+        if (!locate(e, src->x, ib, ii))
+            throw kp::error(KP_ERR_ARG, "source outside the mesh");
+        for (int a = 0; a < 3; ++a) {
+            P.src_x[a] = src->x[a];
+            P.src_i[a] = ii[a];
+            P.src_xl[a] = (float)((src->x[a] - e->xf0_h[3 * ib + a]) / e->dx0_h[3 * ib + a]);
+        }
+        P.src_ib = ib;
+        P.w0 = src->weight;
+        P.seed = run->seed;
+        P.model = run->scatter_model;
+        P.pmax = run->p_max;
+        P.step_max = run->step_max;
+        P.rr_w = run->rr_weight;
+        P.rr_s = run->rr_survive;
+        P.n_incl = run->n_incl;
+        for (int o = 0; o < run->n_incl; ++o) {
+            double s, c;
+            kp_sincos2pi(run->incl_deg[o] / 360.0, &s, &c);
+            P.no[o][0] = s;
+            P.no[o][1] = 0.0;
+            P.no[o][2] = c;
+            P.ev[o][0] = -c;
+            P.ev[o][1] = 0.0;
+            P.ev[o][2] = s;
+            P.eh[o][0] = 0.0;
+            P.eh[o][1] = 1.0;
+            P.eh[o][2] = 0.0;
+        }
+        P.image_n = run->image_n;
+        P.image_half = run->image_half;
+        P.pix_scale = run->image_n > 0 ? (double)run->image_n / (2.0 * run->image_half) : 0.0;
 }
 
 } // namespace
@@ -454,68 +543,8 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         hipStream_t stream = (hipStream_t)stream_;
 
         kp::kparams P;
-        std::memset(&P, 0, sizeof(P));
-        P.xf0 = e->xf0;
-        P.dx0 = e->dx0;
-        P.vol0 = e->vol0;
-        P.nb_id = e->nb_id;
-        P.nb_shift = e->nb_shift;
-        P.d_lvl = e->d_lvl;
-        P.phybnd = e->phybnd;
-        P.h32 = e->h32;
-        P.org32 = e->org32;
-        P.vol32 = e->vol32;
-        P.n_cell = e->n_cell;
-        P.n3 = e->n3;
-        for (int a = 0; a < 3; ++a) {
-            P.periodic[a] = e->periodic[a];
-            P.xmin[a] = e->xmin[a];
-            P.xmax[a] = e->xmax[a];
-            P.center[a] = (e->xmin[a] + e->xmax[a]) * 0.5;
-        }
-        P.kappa = e->kappa;
-        P.albedo = e->albedo;
-        P.g = e->g;
         uint64_t total = 0;
-        upload_bands(e, run, total, stream);
-        P.bands = e->bands_d;
-        P.band_start = e->band_start_d;
-        P.n_bands = run->n_bands;
-        P.total = total;
-        P.packet_offset = run->packet_offset;
-        int ib, ii[3];
-        if (!locate(e, src->x, ib, ii))
-            throw kp::error(KP_ERR_ARG, "kp_forward: source outside the mesh");
-        for (int a = 0; a < 3; ++a) {
-            P.src_x[a] = src->x[a];
-            P.src_i[a] = ii[a];
-            P.src_xl[a] = (float)((src->x[a] - e->xf0_h[3 * ib + a]) / e->dx0_h[3 * ib + a]);
-        }
-        P.src_ib = ib;
-        P.w0 = src->weight;
-        P.seed = run->seed;
-        P.model = run->scatter_model;
-        P.pmax = run->p_max;
-        P.step_max = run->step_max;
-        P.rr_w = run->rr_weight;
-        P.rr_s = run->rr_survive;
-        P.n_incl = run->n_incl;
-        for (int o = 0; o < run->n_incl; ++o) {
-            double s, c;
-            kp_sincos2pi(run->incl_deg[o] / 360.0, &s, &c);
-            P.no[o][0] = s;
-            P.no[o][1] = 0.0;
-            P.no[o][2] = c;
-            P.ev[o][0] = -c;
-            P.ev[o][1] = 0.0;
-            P.ev[o][2] = s;
-            P.eh[o][0] = 0.0;
-            P.eh[o][1] = 1.0;
-            P.eh[o][2] = 0.0;
-        }
-        P.image_n = run->image_n;
-        P.image_half = run->image_half;
-        P.pix_scale = run->image_n > 0 ? (double)run->image_n / (2.0 * run->image_half) : 0.0;
+        fill_params(e, src, run, stream, P, total);
         P.J = out->J;
         P.image = (run->n_incl > 0) ? out->image : nullptr;
         P.counters = (unsigned long long *)out->counters;
@@ -577,6 +606,61 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         if (P.image && P.img_nrep > 1)
             hipLaunchKernelGGL(k_fold, dim3(512), dim3(256), 0, stream, out->image, e->img_rep,
                                nI, P.img_nrep);
+        KP_HIP(hipGetLastError());
+    });
+}
+
+kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
+                     const double *dL_dimage, kp_grads *grads, void *stream_)
+{
+    return kp::guard([&] {
+        if (!e || !src || !run || !dL_dimage || !grads)
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: null argument");
+        if (!e->media_set)
+            throw kp::error(KP_ERR_STATE, "kp_adjoint: kp_set_media not called");
+        if (run->precision != KP_F64)
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_adjoint: path replay runs in FP64 "
+                                                "(precision must be KP_F64)");
+        if (run->n_bands < 1 || !run->bands)
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: need at least one band");
+        if (run->n_incl < 1 || run->n_incl > kp::kMaxIncl || !run->incl_deg)
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: need 1..8 observers");
+        if (run->image_n < 1 || !(run->image_half > 0.0))
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: bad image geometry");
+        if (e->periodic[0] || e->periodic[1] || e->periodic[2])
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_adjoint: needs a non-periodic domain");
+        if (run->scatter_model != KP_THOMSON && run->scatter_model != KP_HG_RAYLEIGH)
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: unknown scatter_model");
+        if (!(run->rr_survive > 0.0 && run->rr_survive <= 1.0))
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: rr_survive must be in (0,1]");
+        device_guard dg(e->device);
+        hipStream_t stream = (hipStream_t)stream_;
+        kp::kparams P;
+        uint64_t total = 0;
+        fill_params(e, src, run, stream, P, total);
+        if (total == 0)
+            return;
+        kp::adj::aparams Q;
+        Q.w = dL_dimage;
+        Q.d_logrho = grads->d_logrho;
+        Q.d_alb = grads->d_albedo;
+        Q.d_g = grads->d_g;
+        Q.jlin = grads->jlin;
+        const int T = 256;
+        uint64_t blocks = (uint64_t)e->n_cu * 4;
+        const uint64_t need = (total + T - 1) / T;
+        if (blocks > need)
+            blocks = need;
+        Q.n_threads = blocks * T;
+        Q.cap = (uint32_t)(32 * run->n_incl);
+        const size_t want = (size_t)Q.cap * Q.n_threads;
+        if (e->adj_cap < want) {
+            dfree(e->adj_scratch);
+            e->adj_scratch = dmalloc<double>(want);
+            e->adj_cap = want;
+        }
+        Q.scratch = e->adj_scratch;
+        hipLaunchKernelGGL(k_adjoint, dim3((unsigned)blocks), dim3(T), 0, stream, P, Q);
         KP_HIP(hipGetLastError());
     });
 }

@@ -160,3 +160,30 @@ class Engine:
                                           C.c_void_p(stream.cuda_stream)))
         del keep
         return outputs
+
+    def adjoint(self, source_x, run: RunSpec, dL_dimage, weight=1.0, grads=None, stream=None):
+        """Path-replay gradients of sum(dL_dimage * image) (FP64).
+
+        dL_dimage: device tensor in the image layout.  Returns a dict of device
+        tensors d_logrho, d_albedo, d_g (per cell) and jlin (scalar), added
+        into ``grads`` if given."""
+        t = self._torch
+        dev = t.device("cuda", self.device)
+        if grads is None:
+            grads = {k: t.zeros(self.mesh.n_cells, dtype=t.float64, device=dev)
+                     for k in ("d_logrho", "d_albedo", "d_g")}
+            grads["jlin"] = t.zeros(1, dtype=t.float64, device=dev)
+        r, keep = run.to_c()
+        g = _abi.Grads()
+        for key in ("d_logrho", "d_albedo", "d_g", "jlin"):
+            v = grads.get(key)
+            setattr(g, key, v.data_ptr() if v is not None else None)
+        w = dL_dimage.contiguous()
+        src = source_c(source_x, weight)
+        if stream is None:
+            stream = t.cuda.current_stream(self.device)
+        _abi.check(_abi.lib().kp_adjoint(self._h, C.byref(src), C.byref(r),
+                                          C.c_void_p(w.data_ptr()), C.byref(g),
+                                          C.c_void_p(stream.cuda_stream)))
+        del keep
+        return grads

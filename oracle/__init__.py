@@ -58,6 +58,9 @@ def lib():
         L.or_hg_mu.argtypes = [C.c_double, C.c_double]
         L.or_sincos2pi.restype = None
         L.or_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_adjoint.restype = C.c_int
+        L.or_adjoint.argtypes = [C.POINTER(_abi.MeshDesc), P, P, P, C.POINTER(_abi.Source),
+                                 C.POINTER(_abi.Run), P, P, P, P, C.POINTER(C.c_double), C.c_int]
         D3 = C.POINTER(C.c_double)
         L.or_scatter_stokes.restype = None
         L.or_scatter_stokes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D3, D3, D3]
@@ -138,3 +141,30 @@ def peel_stokes(model, g, pmax, d, e1, S, no, ev):
                          (C.c_double * 3)(*e1), (C.c_double * 3)(*S), (C.c_double * 3)(*no),
                          (C.c_double * 3)(*ev), out)
     return np.array(out[:])
+
+
+def adjoint(mesh, kappa, albedo, g, source_x, run_spec, dL_dimage, weight=1.0, n_threads=None):
+    """FP64 path-replay adjoint of Jlin = sum(dL_dimage * image): returns
+    (d_logrho, d_albedo, d_g, Jlin) as numpy arrays / float."""
+    from kratos_polrad_amd.engine import source_c
+
+    n = mesh.n_cells
+    k = np.ascontiguousarray(np.asarray(kappa, dtype=np.float32).reshape(n))
+    a = np.ascontiguousarray(np.asarray(albedo, dtype=np.float32).reshape(n))
+    gg = None if g is None else np.ascontiguousarray(np.asarray(g, dtype=np.float32).reshape(n))
+    w = np.ascontiguousarray(np.asarray(dL_dimage, dtype=np.float64))
+    r, keep = run_spec.to_c()
+    dl = np.zeros(n)
+    da = np.zeros(n)
+    dg = np.zeros(n)
+    jl = C.c_double(0.0)
+    src = source_c(source_x, weight)
+    if n_threads is None:
+        n_threads = os.cpu_count() or 1
+    rc = lib().or_adjoint(C.byref(mesh.desc), _ptr(k), _ptr(a), _ptr(gg), C.byref(src),
+                          C.byref(r), _ptr(w), _ptr(dl), _ptr(da), _ptr(dg), C.byref(jl),
+                          int(n_threads))
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"or_adjoint failed: {rc}")
+    return dl, da, dg, jl.value

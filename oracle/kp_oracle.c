@@ -723,3 +723,355 @@ int or_forward(const kp_mesh_desc *m, const float *kappa, const float *albedo, c
     free(c);
     return rc;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* Adjoint by path replay                                              */
+/*
+ * Gradient of the linear image functional Jlin = sum_pix w_pix . S_pix
+ * (w = dL/dimage, Stokes I,Q,U per pixel) with respect to per-cell
+ * log-density (kappa_rho scales with rho), albedo and HG g.  The image
+ * estimator is the path integral f/p over sampled histories, so
+ * dE[Jlin] = E[(f/p) dlog f]: for the peel-off contribution A_k of event k,
+ *   dlog f/dlog rho_c = (#vertices j<=k in c) - tau_c(main path up to x_k)
+ *                       - tau_c(peel-off ray k),
+ *   dlog f/d a_c      = (#vertices j<=k in c) / a_c,
+ *   dlog f/d g_c      = sum_{scatterings j<k in c} dlogP_HG(mu_j)/dg
+ *                       + [x_k in c] dlogP_HG(mu_obs,k)/dg.
+ * Path replay backpropagation (Vicini, Speierer & Jakob 2021): pass 1 replays
+ * the history to get every A_k; pass 2 replays it again carrying the suffix
+ * sum B_j = sum_{k>=j} A_k, so each segment, vertex and scattering adds its
+ * term once.  Russian roulette adds no term (its probability is not
+ * differentiated; the estimator stays unbiased).
+ */
+
+typedef struct {
+    const double *w;          /* dL/dimage, image layout */
+    double *d_logrho, *d_alb, *d_g;
+    double jlin;
+    double *A;                /* A per (event, observer) of the current packet */
+    size_t A_cap;
+} athr_t;
+
+static double dlogp_hg(double gg, double mu)
+{
+    const double D = (1.0 + gg * gg) - 2.0 * gg * mu;
+    return -2.0 * gg / (1.0 - gg * gg) - 3.0 * (gg - mu) / D;
+}
+
+/* pixel index of x for observer o, or -1 */
+static long pix_index(const fctx_t *c, int band, int o, const double *x)
+{
+    const int N = c->run->image_n;
+    double rel[3];
+    for (int a = 0; a < 3; ++a)
+        rel[a] = x[a] - c->center[a];
+    const double uh = dot3(rel, c->eh[o]);
+    const double uv = dot3(rel, c->ev[o]);
+    const double fx = floor((uh + c->run->image_half) * c->pix_scale);
+    const double fy = floor((uv + c->run->image_half) * c->pix_scale);
+    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+        return -1;
+    return (long)((((size_t)band * c->n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
+}
+
+/* peel-off ray gradient: d_logrho[c] -= kappa_c l_c A along the ray, with the
+ * same march and the same early stop as tau_to_edge */
+static void peel_grad(const fctx_t *c, const ray_t *from, const double *dir, double ks, double A,
+                      double *d_logrho)
+{
+    const kp_mesh_desc *m = c->m;
+    const int n3 = m->n_cell * m->n_cell * m->n_cell;
+    ray_t r = *from;
+    memcpy(r.d, dir, sizeof(r.d));
+    double tau = 0.0;
+    for (;;) {
+        double l;
+        const int ap = get_dl(&r, &l);
+        if (ap < 0)
+            break;
+        const size_t gc = (size_t)r.ib * n3 + cell_index(m, &r);
+        const double kap = (double)c->kappa[gc] * ks;
+        tau = tau + kap * l;
+        d_logrho[gc] -= (kap * l) * A;
+        if (tau > TAU_CUT)
+            break;
+        int reload;
+        const int st = cross_face(m, &r, ap, l, &reload);
+        if (st != ST_IN)
+            break;
+        if (reload)
+            gl_load(m, &r);
+    }
+}
+
+static void adjoint_packet(const fctx_t *c, athr_t *t, int band, uint64_t pid)
+{
+    const kp_mesh_desc *m = c->m;
+    const kp_run *run = c->run;
+    const kp_band *bd = &run->bands[band];
+    const int n3 = m->n_cell * m->n_cell * m->n_cell;
+    const double ks = bd->kappa_scale;
+    const double w0 = c->src->weight;
+    const uint32_t k0 = run->seed, k1 = (uint32_t)band;
+    const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
+    const int no_ = c->n_incl;
+    const int hg = run->scatter_model == KP_HG_RAYLEIGH && (bd->flags & KP_BAND_CELL_G);
+    const int calb = (bd->flags & KP_BAND_CELL_ALBEDO) != 0;
+    double A_tot = 0.0;
+    uint64_t dummy = 0;
+
+    for (int pass = 1; pass <= 2; ++pass) {
+        uint32_t rn[4] = {plo, phi, 0u, 0u};
+        kp_philox4x32_10(rn, k0, k1);
+        ray_t r;
+        double e1[3];
+        {
+            const double mu = fma(2.0, kp_u01(rn[0]), -1.0);
+            double sf, cf;
+            kp_sincos2pi(kp_u01(rn[1]), &sf, &cf);
+            double s2 = fma(-mu, mu, 1.0);
+            if (s2 < 0.0) s2 = 0.0;
+            const double st = sqrt(s2);
+            r.d[0] = st * cf;
+            r.d[1] = st * sf;
+            r.d[2] = mu;
+            e1[0] = mu * cf;
+            e1[1] = mu * sf;
+            e1[2] = -st;
+        }
+        double tau_rem = -kp_log(kp_u01(rn[2]));
+        double I = w0, Q = 0.0, U = 0.0;
+        memcpy(r.x, c->src->x, sizeof(r.x));
+        r.ib = c->src_ib;
+        memcpy(r.i, c->src_i, sizeof(r.i));
+        gl_load(m, &r);
+        double B = 0.0;
+        /* event 0: emission peel-offs */
+        for (int o = 0; o < no_; ++o) {
+            if (pass == 1) {
+                const double tau = tau_to_edge(c, &r, c->no[o], ks, &dummy);
+                const double att = kp_exp(-tau) * KP_INV_4PI;
+                const long px = pix_index(c, band, o, r.x);
+                double Ak = 0.0;
+                if (px >= 0)
+                    Ak = t->w[3 * px] * (I * att);
+                t->A[o] = Ak;
+                A_tot += Ak;
+            } else {
+                peel_grad(c, &r, c->no[o], ks, t->A[o], t->d_logrho);
+            }
+        }
+        if (pass == 2) {
+            B = A_tot;
+            for (int o = 0; o < no_; ++o)
+                B -= t->A[o];
+        }
+        int steps = run->step_max;
+        int events = 0;
+        for (;;) {
+            int interact = 0;
+            size_t gc = 0;
+            for (;;) {
+                if (steps <= 0)
+                    break;
+                double l;
+                const int ap = get_dl(&r, &l);
+                if (ap < 0)
+                    break;
+                gc = (size_t)r.ib * n3 + cell_index(m, &r);
+                const double kap = (double)c->kappa[gc] * ks;
+                const double dtau = kap * l;
+                steps -= 1;
+                if (kap > 0.0 && dtau >= tau_rem) {
+                    double li = tau_rem / kap;
+                    if (li > l)
+                        li = l;
+                    if (pass == 2)
+                        t->d_logrho[gc] -= (kap * li) * B;
+                    for (int a = 0; a < 3; ++a)
+                        r.x[a] = r.x[a] + li * r.d[a];
+                    interact = 1;
+                    break;
+                }
+                tau_rem = tau_rem - dtau;
+                if (pass == 2)
+                    t->d_logrho[gc] -= dtau * B;
+                int reload;
+                const int st = cross_face(m, &r, ap, l, &reload);
+                if (st != ST_IN)
+                    break;
+                if (reload)
+                    gl_load(m, &r);
+            }
+            if (!interact)
+                break;
+            events += 1;
+            uint32_t u[4] = {plo, phi, (uint32_t)events, 0u};
+            kp_philox4x32_10(u, k0, k1);
+            const double alb = calb ? (double)c->albedo[gc] : bd->albedo;
+            const double gg = (bd->flags & KP_BAND_CELL_G) ? (c->g ? (double)c->g[gc] : 0.0) : bd->g;
+            I = I * alb;
+            Q = Q * alb;
+            U = U * alb;
+            if (pass == 2) {
+                t->d_logrho[gc] += B;
+                if (calb && t->d_alb)
+                    t->d_alb[gc] += B / alb;
+            }
+            double e2[3];
+            cross3(r.d, e1, e2);
+            if ((size_t)(events + 1) * no_ > t->A_cap) {
+                t->A_cap = 2 * (size_t)(events + 1) * no_ + 16;
+                t->A = (double *)realloc(t->A, t->A_cap * sizeof(double));
+            }
+            for (int o = 0; o < no_; ++o) {
+                double S[3] = {I, Q, U}, Si[3];
+                peel_stokes(run->scatter_model, gg, run->p_max, r.d, e1, e2, S, c->no[o],
+                            c->ev[o], Si);
+                double *Ak = &t->A[(size_t)events * no_ + o];
+                if (pass == 1) {
+                    const double tau = tau_to_edge(c, &r, c->no[o], ks, &dummy);
+                    const double att = kp_exp(-tau) * KP_INV_4PI;
+                    const long px = pix_index(c, band, o, r.x);
+                    double a = 0.0;
+                    if (px >= 0)
+                        a = t->w[3 * px] * (Si[0] * att) + t->w[3 * px + 1] * (Si[1] * att) +
+                            t->w[3 * px + 2] * (Si[2] * att);
+                    *Ak = a;
+                    A_tot += a;
+                } else {
+                    peel_grad(c, &r, c->no[o], ks, *Ak, t->d_logrho);
+                    if (hg && t->d_g)
+                        t->d_g[gc] += *Ak * dlogp_hg(gg, fmin(fmax(dot3(r.d, c->no[o]), -1.0), 1.0));
+                    B -= *Ak;
+                }
+            }
+            const double mu = (run->scatter_model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
+                                                                  : kp_hg_mu(gg, kp_u01(u[0]));
+            if (pass == 2 && hg && t->d_g)
+                t->d_g[gc] += B * dlogp_hg(gg, mu);
+            double S[3] = {I, Q, U};
+            scatter_stokes(run->scatter_model, run->p_max, mu, kp_u01(u[1]), r.d, e1, e2, S);
+            I = S[0];
+            Q = S[1];
+            U = S[2];
+            tau_rem = -kp_log(kp_u01(u[2]));
+            if (I < run->rr_weight * w0) {
+                if (kp_u01(u[3]) < run->rr_survive) {
+                    I = I / run->rr_survive;
+                    Q = Q / run->rr_survive;
+                    U = U / run->rr_survive;
+                } else {
+                    break;
+                }
+            }
+        }
+    }
+    t->jlin += A_tot;
+}
+
+typedef struct {
+    const fctx_t *c;
+    uint64_t k_lo, k_hi;
+    athr_t a;
+} awork_t;
+
+static void *aworker(void *arg)
+{
+    awork_t *w = (awork_t *)arg;
+    const fctx_t *c = w->c;
+    int band = 0;
+    for (uint64_t k = w->k_lo; k < w->k_hi; ++k) {
+        while (k >= c->band_start[band + 1])
+            ++band;
+        const uint64_t pid = c->run->packet_offset + (k - c->band_start[band]);
+        adjoint_packet(c, &w->a, band, pid);
+    }
+    return NULL;
+}
+
+int or_adjoint(const kp_mesh_desc *m, const float *kappa, const float *albedo, const float *g,
+               const kp_source *src, const kp_run *run, const double *dL_dimage,
+               double *d_logrho, double *d_albedo, double *d_g, double *jlin, int n_threads)
+{
+    if (!m || !kappa || !src || !run || !dL_dimage || !d_logrho || run->n_bands < 1 ||
+        run->n_bands > 1023 || run->n_incl < 1 || run->n_incl > 8)
+        return -1;
+    fctx_t *c = (fctx_t *)calloc(1, sizeof(fctx_t));
+    c->m = m;
+    c->kappa = kappa;
+    c->albedo = albedo;
+    c->g = g;
+    c->src = src;
+    c->run = run;
+    if (or_locate(m, src->x, &c->src_ib, c->src_i) != 0) {
+        free(c);
+        return -2;
+    }
+    c->n_incl = run->n_incl;
+    for (int o = 0; o < run->n_incl; ++o) {
+        double s, co;
+        kp_sincos2pi(run->incl_deg[o] / 360.0, &s, &co);
+        c->no[o][0] = s;  c->no[o][1] = 0.0; c->no[o][2] = co;
+        c->ev[o][0] = -co; c->ev[o][1] = 0.0; c->ev[o][2] = s;
+        c->eh[o][0] = 0.0; c->eh[o][1] = 1.0; c->eh[o][2] = 0.0;
+    }
+    for (int a = 0; a < 3; ++a)
+        c->center[a] = (m->x_min[a] + m->x_max[a]) * 0.5;
+    c->pix_scale = (double)run->image_n / (2.0 * run->image_half);
+    c->band_start[0] = 0;
+    for (int b = 0; b < run->n_bands; ++b)
+        c->band_start[b + 1] = c->band_start[b] + run->bands[b].n_packets;
+    c->total = c->band_start[run->n_bands];
+    for (int b = run->n_bands + 1; b < 1024; ++b)
+        c->band_start[b] = c->total;
+    if (n_threads < 1)
+        n_threads = 1;
+    if ((uint64_t)n_threads > c->total && c->total > 0)
+        n_threads = (int)c->total;
+    const size_t ncell = (size_t)m->n_blocks * m->n_cell * m->n_cell * m->n_cell;
+    awork_t *w = (awork_t *)calloc((size_t)n_threads, sizeof(awork_t));
+    pthread_t *tid = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+    for (int k = 0; k < n_threads; ++k) {
+        w[k].c = c;
+        w[k].k_lo = c->total * k / n_threads;
+        w[k].k_hi = c->total * (k + 1) / n_threads;
+        w[k].a.w = dL_dimage;
+        w[k].a.d_logrho = k == 0 ? d_logrho : (double *)calloc(ncell, sizeof(double));
+        w[k].a.d_alb = d_albedo ? (k == 0 ? d_albedo : (double *)calloc(ncell, sizeof(double))) : NULL;
+        w[k].a.d_g = d_g ? (k == 0 ? d_g : (double *)calloc(ncell, sizeof(double))) : NULL;
+        w[k].a.A_cap = 64;
+        w[k].a.A = (double *)malloc(64 * sizeof(double));
+    }
+    for (int k = 1; k < n_threads; ++k)
+        pthread_create(&tid[k], NULL, aworker, &w[k]);
+    aworker(&w[0]);
+    for (int k = 1; k < n_threads; ++k)
+        pthread_join(tid[k], NULL);
+    double jl = 0.0;
+    for (int k = 0; k < n_threads; ++k) {
+        jl += w[k].a.jlin;
+        if (k > 0) {
+            for (size_t q = 0; q < ncell; ++q) {
+                d_logrho[q] += w[k].a.d_logrho[q];
+                if (d_albedo)
+                    d_albedo[q] += w[k].a.d_alb[q];
+                if (d_g)
+                    d_g[q] += w[k].a.d_g[q];
+            }
+            free(w[k].a.d_logrho);
+            if (d_albedo)
+                free(w[k].a.d_alb);
+            if (d_g)
+                free(w[k].a.d_g);
+        }
+        free(w[k].a.A);
+    }
+    if (jlin)
+        *jlin += jl;
+    free(w);
+    free(tid);
+    free(c);
+    return 0;
+}

@@ -42,6 +42,15 @@ int or_forward(const kp_mesh_desc *mesh, const float *kappa, const float *albedo
                const float *g, const kp_source *src, const kp_run *run, double *J,
                double *image, uint64_t *counters, double *end_state, int n_threads);
 
+/* Adjoint by path replay of the linear image functional
+ * Jlin = sum_pix dL_dimage[pix] . image[pix] (image layout, I,Q,U):
+ * accumulates dJlin/dlog(rho_c), dJlin/d(albedo_c), dJlin/d(g_c) per cell
+ * (d_albedo / d_g may be NULL) and adds the primal Jlin to *jlin.  FP64. */
+int or_adjoint(const kp_mesh_desc *mesh, const float *kappa, const float *albedo,
+               const float *g, const kp_source *src, const kp_run *run,
+               const double *dL_dimage, double *d_logrho, double *d_albedo, double *d_g,
+               double *jlin, int n_threads);
+
 /* Pure-streaming trace of one ray (the reference photon.h:104-125 march with
  * no medium): records every step's block, in-block cell index and length.
  * Returns the number of steps written (<= max_steps); *status is 1 when
