@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--skip-f64", action="store_true")
+    p.add_argument("--skip-adjoint", action="store_true")
+    p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 512 MiB device-to-device copy (3 times) so "
                         "rocprofv3 counters can be scaled by a kernel of known traffic")
@@ -64,6 +66,66 @@ def img_err(a, b):
         x, y = a[..., k].ravel(), b[..., k].ravel()
         out[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(b[..., 0].ravel()), 1e-300))
     return out
+
+
+def adjoint_ratio(n):
+    """Adjoint/forward time on the 2-level adjoint workload: loss = squared Q/U
+    residuals against an image of the perturbed medium (other packet ids)."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi, configs
+    from kratos_polrad_amd.engine import Engine, RunSpec
+
+    aw = configs.config4(n_packets=n)
+    w = aw.base
+    eng = Engine(w.mesh)
+    run64 = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+    run32 = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F32})
+    # observed image: perturbed medium, independent packets
+    eng.set_media(aw.kappa_obs, w.albedo, w.g)
+    obs_run = RunSpec(**{**run64.__dict__, "packet_offset": 1 << 40})
+    obs = eng.alloc_outputs(obs_run, J=False)
+    eng.forward(w.source, obs_run, obs)
+    eng.set_media(w.kappa, w.albedo, w.g)
+
+    def tfwd(run):
+        out = eng.alloc_outputs(run, J=True)
+        eng.forward(w.source, run, out)  # warm-up launch
+        torch.cuda.synchronize()
+        for v in out.values():
+            v.zero_()
+        t0 = time.perf_counter()
+        eng.forward(w.source, run, out)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    t64, out64 = tfwd(run64)
+    t32, _ = tfwd(run32)
+    wgt = torch.zeros_like(out64["image"])
+    wgt[..., 1:] = 2.0 * (out64["image"][..., 1:] - obs["image"][..., 1:])
+    loss = float(((out64["image"][..., 1:] - obs["image"][..., 1:]) ** 2).sum())
+
+    def tadj(run):
+        eng.adjoint(w.source, run, wgt)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gr = eng.adjoint(w.source, run, wgt)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, gr
+
+    ta64, gr64 = tadj(run64)
+    ta32, gr32 = tadj(run32)
+    return {"workload": w.name, "packets": n, "blocks": w.mesh.n_blocks,
+            "cells": w.mesh.n_cells, "loss": loss,
+            "forward_fp64_ms": 1e3 * t64, "forward_mixed_ms": 1e3 * t32,
+            "adjoint_fp64_ms": 1e3 * ta64, "adjoint_mixed_ms": 1e3 * ta32,
+            "adjoint_over_forward_fp64": ta64 / t64,
+            "adjoint_over_forward_mixed": ta32 / t32,
+            "grad_norms_fp64": {k: float(torch.linalg.vector_norm(v))
+                                for k, v in gr64.items() if k != "jlin"},
+            "grad_norms_mixed": {k: float(torch.linalg.vector_norm(v))
+                                 for k, v in gr32.items() if k != "jlin"}}
 
 
 def main():
@@ -166,6 +228,9 @@ def main():
             "mixed_vs_independent_fp64": img_err(img32, ref),
             "fp64_vs_independent_fp64": img_err(img64, ref),
         }
+
+    if not args.skip_adjoint:
+        res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))
 
     if not args.no_cpu:
         import oracle

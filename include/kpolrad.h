@@ -240,9 +240,11 @@ typedef struct kp_grads {
 
 /* Gradient of Jlin = sum_pix dL_dimage[pix] . image[pix] (device array in the
  * kp_tallies image layout) by path replay: every history of the run is
- * replayed twice from its Philox key, FP64 transport and accumulation.
- * Same packets as kp_forward with the same run.  No reference counterpart
- * (the reference has no adjoint); it serves image-fitting callers. */
+ * replayed twice from its Philox key in the run's precision (KP_F64, or
+ * KP_F32 with the persistent schedule: the histories of the mixed-precision
+ * forward), with FP64 accumulation of A_k, B and the gradients.  Same packets
+ * as kp_forward with the same run.  No reference counterpart (the reference
+ * has no adjoint); it serves image-fitting callers. */
 kp_status kp_adjoint(kp_engine *eng, const kp_source *src, const kp_run *run,
                      const double *dL_dimage, kp_grads *grads, void *stream);
 

@@ -10,6 +10,11 @@ Config 2: 3-level block AMR, rho ~ r^-2 times a lognormal field (sigma 0.5,
 seeded Gaussian random field with a k^-11/3 power spectrum), refinement of
 every block whose max |grad rho| dx / rho exceeds 0.5, HG g=0.6, albedo 0.5,
 Rayleigh-form linear polarization with p_max 0.5.
+
+Config 4 (adjoint): the config-2 field on a 2-level mesh of 3^3 root blocks
+(96^3 effective); the "observed" Q/U image is rendered from the same field
+times an extra seeded lognormal factor (sigma 0.2); loss = sum of squared Q
+and U residuals.
 """
 from __future__ import annotations
 
@@ -167,3 +172,31 @@ def config2(n_packets: int = 100_000_000, precision: int = _abi.KP_F32, seed: in
                   scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
                   incl_deg=(90.0,), image_n=128, image_half=0.5)
     return Workload("config2_amr3_r2_lognormal_hg", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
+
+
+def lognormal_factor(x: np.ndarray, sigma: float, seed: int, field_n: int = 112) -> np.ndarray:
+    """exp(sigma G - sigma^2/2) of a unit-variance k^-11/3 random field at x."""
+    grf = gaussian_random_field(field_n, seed)
+    return np.exp(sigma * _sample_field(grf, x) - 0.5 * sigma ** 2)
+
+
+@dataclass
+class AdjointWorkload:
+    base: Workload
+    kappa_obs: np.ndarray  # extinction of the perturbed ("observed") medium
+
+
+def config4(n_packets: int = 10_000_000, seed: int = 42, tau_centre: float = 5.0,
+            obs_sigma: float = 0.2, obs_seed: int = 11) -> AdjointWorkload:
+    mesh, rho, rho_at = amr_density(root=3, levels=2)
+    k0 = tau_centre / centre_tau_norm(rho_at)
+    kappa = (rho * k0).astype(np.float32)
+    kappa_obs = (rho * k0 * lognormal_factor(mesh.cell_centers(), obs_sigma, obs_seed)
+                 ).astype(np.float32)
+    albedo = np.full(mesh.n_cells, 0.5, dtype=np.float32)
+    g = np.full(mesh.n_cells, 0.6, dtype=np.float32)
+    run = RunSpec(bands=[BandSpec(n_packets=n_packets, cell_albedo=True, cell_g=True)],
+                  seed=seed, scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5,
+                  precision=_abi.KP_F64, incl_deg=(90.0,), image_n=128, image_half=0.5)
+    base = Workload("config4_amr2_96_adjoint", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
+    return AdjointWorkload(base, kappa_obs)

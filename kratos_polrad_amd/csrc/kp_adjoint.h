@@ -33,26 +33,10 @@ struct aparams {
     uint64_t n_threads;
 };
 
-__device__ __forceinline__ double dlogp_hg(double gg, double mu)
+__device__ __forceinline__ double dlogp_hg_h(double gg, double mu)
 {
     const double D = (1.0 + gg * gg) - 2.0 * gg * mu;
     return -2.0 * gg / (1.0 - gg * gg) - 3.0 * (gg - mu) / D;
-}
-
-__device__ __forceinline__ long pix_index(const kparams &P, int band, int o, const double *x)
-{
-    const int N = P.image_n;
-    double rel[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        rel[a] = x[a] - P.center[a];
-    const double uh = f64::dot3(rel, P.eh[o]);
-    const double uv = f64::dot3(rel, P.ev[o]);
-    const double fx = floor((uh + P.image_half) * P.pix_scale);
-    const double fy = floor((uv + P.image_half) * P.pix_scale);
-    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
-        return -1;
-    return (long)((((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
 }
 
 __device__ void peel_grad(const kparams &P, const f64::ray_t &from, const double *dir, double ks,
@@ -92,7 +76,7 @@ __device__ __forceinline__ double peel_A(const kparams &P, const aparams &Q, con
     uint64_t dummy = 0;
     const double tau = f64::tau_to_edge(P, r, P.no[o], ks, dummy);
     const double att = kp_exp(-tau) * KP_INV_4PI;
-    const long px = pix_index(P, band, o, r.x);
+    const long px = f64::pixel(P, band, o, r.x);
     if (px < 0)
         return 0.0;
     return Q.w[3 * px] * (I * att) + Q.w[3 * px + 1] * (Qs * att) + Q.w[3 * px + 2] * (U * att);
@@ -224,7 +208,7 @@ __device__ void adjoint_packet(const kparams &P, const aparams &Q, int band, uin
                     if (hg && Q.d_g && a != 0.0) {
                         double mo = f64::dot3(r.d, P.no[o]);
                         mo = fmin(fmax(mo, -1.0), 1.0);
-                        atomic_add_f64(Q.d_g + gc, a * dlogp_hg(gg, mo));
+                        atomic_add_f64(Q.d_g + gc, a * dlogp_hg_h(gg, mo));
                     }
                     B -= a;
                 }
@@ -232,7 +216,7 @@ __device__ void adjoint_packet(const kparams &P, const aparams &Q, int band, uin
             const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
                                                       : kp_hg_mu(gg, kp_u01(u[0]));
             if (pass == 2 && hg && Q.d_g && B != 0.0)
-                atomic_add_f64(Q.d_g + gc, B * dlogp_hg(gg, mu));
+                atomic_add_f64(Q.d_g + gc, B * dlogp_hg_h(gg, mu));
             f64::scatter_stokes(P.model, P.pmax, mu, kp_u01(u[1]), r.d, e1, e2, I, Qs, U);
             tau_rem = -kp_log(kp_u01(u[2]));
             if (I < P.rr_w * w0) {

@@ -80,7 +80,21 @@ struct kparams {
     // replica b % R, spreading FP64 atomics on hot cells/pixels
     uint32_t j_nrep, img_nrep;
     uint64_t j_stride, img_stride; // elements per replica
+    // adjoint (path replay) launches
+    const double *adj_w;           // dL/dimage, image layout
+    double *g_logrho, *g_alb, *g_g; // gradient replica bases (may be null)
+    uint32_t g_nrep;
+    uint64_t g_stride;             // cells per gradient replica
+    double *adj_rows;              // [adj_cap][adj_lanes] peel-off contributions A_k
+    uint32_t adj_cap;
+    uint64_t adj_lanes;
+    double *jlin;                  // device scalar
 };
+
+__device__ __forceinline__ double *g_base(const kparams &P, double *base)
+{
+    return base + (uint64_t)(blockIdx.x % P.g_nrep) * P.g_stride;
+}
 
 __device__ __forceinline__ double *j_base(const kparams &P)
 {

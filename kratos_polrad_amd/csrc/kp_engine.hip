@@ -54,9 +54,11 @@ struct kp_engine {
     // tally / image replicas (zero between launches)
     double *j_rep = nullptr, *img_rep = nullptr;
     size_t j_rep_cap = 0, img_rep_cap = 0; // elements
-    // adjoint: per-thread rows of peel-off contributions
+    // adjoint: per-thread rows of peel-off contributions, gradient replicas
     double *adj_scratch = nullptr;
     size_t adj_cap = 0;
+    double *g_rep = nullptr;
+    size_t g_rep_cap = 0;
 };
 
 namespace {
@@ -175,32 +177,41 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
         p[e] = 0.0;
 }
 
-template <class Pol, int WPS>
+template <class Pol, int WPS, bool ADJ>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
     kp::lane_counters C;
-    kp::persist_body<Pol>(P, P.work, C);
+    double jsum = 0.0;
+    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum);
     flush_counters(P, C);
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            jsum += __shfl_xor(jsum, off, 64);
+        if ((threadIdx.x & 63) == 0 && P.jlin && jsum != 0.0)
+            atomicAdd(P.jlin, jsum);
+    }
 }
 
 // register-budget variants of the persistent kernel
 using persist_fn = void (*)(kp::kparams);
-persist_fn persist_variant(int precision, int wps)
+template <bool ADJ> persist_fn persist_variant(int precision, int wps)
 {
     if (precision == KP_F64) {
         if (wps == 3)
-            return k_persist<kp::pol64, 3>;
+            return k_persist<kp::pol64, 3, ADJ>;
         if (wps == 4)
-            return k_persist<kp::pol64, 4>;
-        return k_persist<kp::pol64, 1>;
+            return k_persist<kp::pol64, 4, ADJ>;
+        return k_persist<kp::pol64, 1, ADJ>;
     }
     if (wps == 5)
-        return k_persist<kp::pol32, 5>;
+        return k_persist<kp::pol32, 5, ADJ>;
     if (wps == 6)
-        return k_persist<kp::pol32, 6>;
-    return k_persist<kp::pol32, 1>;
+        return k_persist<kp::pol32, 6, ADJ>;
+    return k_persist<kp::pol32, 1, ADJ>;
 }
 
+// One thread per packet, both replay passes in that thread (FP64).
 __global__ void __launch_bounds__(256) k_adjoint(kp::kparams P, kp::adj::aparams Q)
 {
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,6 +250,8 @@ void free_all(kp_engine *e)
     dfree(e->img_rep);
     dfree(e->adj_scratch);
     e->adj_cap = 0;
+    dfree(e->g_rep);
+    e->g_rep_cap = 0;
     e->j_rep_cap = e->img_rep_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
@@ -596,7 +609,7 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
             P.chunk = (uint32_t)S.chunk;
             P.batch = (uint32_t)S.event_batch;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
-            hipLaunchKernelGGL(persist_variant(run->precision, S.waves_per_simd),
+            hipLaunchKernelGGL(persist_variant<false>(run->precision, S.waves_per_simd),
                                dim3((unsigned)blocks), dim3(T), 0, stream, P);
         }
         KP_HIP(hipGetLastError());
@@ -618,9 +631,11 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
             throw kp::error(KP_ERR_ARG, "kp_adjoint: null argument");
         if (!e->media_set)
             throw kp::error(KP_ERR_STATE, "kp_adjoint: kp_set_media not called");
-        if (run->precision != KP_F64)
-            throw kp::error(KP_ERR_UNSUPPORTED, "kp_adjoint: path replay runs in FP64 "
-                                                "(precision must be KP_F64)");
+        if (run->precision != KP_F64 && run->precision != KP_F32)
+            throw kp::error(KP_ERR_ARG, "kp_adjoint: unknown precision");
+        if (run->precision == KP_F32 && e->sched.kind == KP_SCHED_HISTORY)
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_adjoint: the one-thread-per-packet "
+                                                "schedule replays FP64 histories only");
         if (run->n_bands < 1 || !run->bands)
             throw kp::error(KP_ERR_ARG, "kp_adjoint: need at least one band");
         if (run->n_incl < 1 || run->n_incl > kp::kMaxIncl || !run->incl_deg)
@@ -640,28 +655,73 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         fill_params(e, src, run, stream, P, total);
         if (total == 0)
             return;
-        kp::adj::aparams Q;
-        Q.w = dL_dimage;
-        Q.d_logrho = grads->d_logrho;
-        Q.d_alb = grads->d_albedo;
-        Q.d_g = grads->d_g;
-        Q.jlin = grads->jlin;
-        const int T = 256;
-        uint64_t blocks = (uint64_t)e->n_cu * 4;
+        const kp_schedule &S = e->sched;
+        const int T = S.kind == KP_SCHED_HISTORY ? 256 : S.block_threads;
+        uint64_t blocks = (uint64_t)e->n_cu * (S.kind == KP_SCHED_HISTORY ? 4 : S.blocks_per_cu);
         const uint64_t need = (total + T - 1) / T;
         if (blocks > need)
             blocks = need;
-        Q.n_threads = blocks * T;
-        Q.cap = (uint32_t)(32 * run->n_incl);
-        const size_t want = (size_t)Q.cap * Q.n_threads;
+        const uint64_t lanes = blocks * T;
+        const uint32_t cap = (uint32_t)(32 * run->n_incl);
+        const size_t want = (size_t)cap * lanes;
         if (e->adj_cap < want) {
             dfree(e->adj_scratch);
             e->adj_scratch = dmalloc<double>(want);
             e->adj_cap = want;
         }
-        Q.scratch = e->adj_scratch;
-        hipLaunchKernelGGL(k_adjoint, dim3((unsigned)blocks), dim3(T), 0, stream, P, Q);
+        if (S.kind == KP_SCHED_HISTORY) {
+            kp::adj::aparams Q;
+            Q.w = dL_dimage;
+            Q.d_logrho = grads->d_logrho;
+            Q.d_alb = grads->d_albedo;
+            Q.d_g = grads->d_g;
+            Q.jlin = grads->jlin;
+            Q.n_threads = lanes;
+            Q.cap = cap;
+            Q.scratch = e->adj_scratch;
+            hipLaunchKernelGGL(k_adjoint, dim3((unsigned)blocks), dim3(T), 0, stream, P, Q);
+            KP_HIP(hipGetLastError());
+            return;
+        }
+        // persistent path replay with replicated gradient accumulators
+        const uint64_t nc = (uint64_t)e->n_blocks * e->n3;
+        P.adj_w = dL_dimage;
+        P.adj_rows = e->adj_scratch;
+        P.adj_cap = cap;
+        P.adj_lanes = lanes;
+        P.jlin = grads->jlin;
+        P.g_stride = nc;
+        P.g_nrep = pick_replicas(S.j_replicas, 3 * nc, blocks, 160ull << 20);
+        double *gl = grads->d_logrho, *ga = grads->d_albedo, *gg = grads->d_g;
+        if (P.g_nrep > 1) {
+            ensure_replicas(e->g_rep, e->g_rep_cap, 3ull * P.g_nrep * nc, stream);
+            P.g_logrho = gl ? e->g_rep : nullptr;
+            P.g_alb = ga ? e->g_rep + (uint64_t)P.g_nrep * nc : nullptr;
+            P.g_g = gg ? e->g_rep + 2ull * P.g_nrep * nc : nullptr;
+        } else {
+            P.g_logrho = gl;
+            P.g_alb = ga;
+            P.g_g = gg;
+        }
+        P.work = e->work;
+        P.chunk = (uint32_t)S.chunk;
+        P.batch = (uint32_t)S.event_batch;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
+        hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
+                           dim3((unsigned)blocks), dim3(T), 0, stream, P);
         KP_HIP(hipGetLastError());
+        if (P.g_nrep > 1) {
+            if (gl)
+                hipLaunchKernelGGL(k_fold, dim3(1024), dim3(256), 0, stream, gl, P.g_logrho, nc,
+                                   P.g_nrep);
+            if (ga)
+                hipLaunchKernelGGL(k_fold, dim3(1024), dim3(256), 0, stream, ga, P.g_alb, nc,
+                                   P.g_nrep);
+            if (gg)
+                hipLaunchKernelGGL(k_fold, dim3(1024), dim3(256), 0, stream, gg, P.g_g, nc,
+                                   P.g_nrep);
+            KP_HIP(hipGetLastError());
+        }
     });
 }
 

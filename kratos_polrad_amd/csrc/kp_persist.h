@@ -138,17 +138,26 @@ template <class Pol> struct lane_t {
     double tau;       // MAIN: optical depth left; PEEL: accumulated
     double tau_main;
     real gg;          // HG asymmetry at the current interaction
+    real alb;         // albedo at the current interaction
     uint32_t u[4];    // Philox words of the current interaction
     uint64_t slot, pid;
     int band;
     int mode, ev_kind, o;
     int pa, status;   // pending action and, for PA_FINISH, the end status
-    int64_t gc;       // cell of the pending interaction
+    int64_t gc;       // cell of the pending / current interaction
     int64_t kgc;      // cell whose kappa_rho is in kcache (issued one step ahead)
     float kcache;
     int events, steps;
     uint32_t ncross, npeel;
     f32::tally_acc acc; // FP32: Kahan register accumulator of the current cell
+    // path replay (adjoint launches only)
+    int pass;         // 1: collect peel-off contributions A_k; 2: gradients
+    int peel_sub;     // pass 2: 0 = re-measuring an A_k beyond the row, 1 = depositing
+    uint32_t kslot;   // next A_k slot of this packet
+    double Atot;      // sum of the packet's A_k
+    double B;         // pass 2: sum of the A_k still ahead
+    double Acur;      // A_k of the current peel-off ray
+    double jsum;      // lane's sum of A_k over its packets
 };
 
 struct lane_counters {
@@ -156,15 +165,33 @@ struct lane_counters {
              packets = 0;
 };
 
-template <class Pol>
+__device__ __forceinline__ double dlogp_hg(double gg, double mu)
+{
+    const double D = (1.0 + gg * gg) - 2.0 * gg * mu;
+    return -2.0 * gg / (1.0 - gg * gg) - 3.0 * (gg - mu) / D;
+}
+
+template <class Pol, bool ADJ> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L);
+
+template <class Pol, bool ADJ>
 __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_counters &C,
                                        int status)
 {
-    if constexpr (sizeof(typename Pol::real) == 4)
+    if constexpr (!ADJ && sizeof(typename Pol::real) == 4)
         L.acc.flush(P, L.band);
     C.cross += L.ncross;
     C.peel += L.npeel;
     C.events += (uint32_t)L.events;
+    if constexpr (ADJ) {
+        if (L.pass == 1) {
+            // replay the same history for the gradients
+            L.pass = 2;
+            emit_packet<Pol, ADJ>(P, L);
+            return;
+        }
+        L.jsum += L.Atot;
+        L.pass = 1;
+    }
     C.packets += 1;
     if (status == KP_END_ESCAPED)
         C.escaped += 1;
@@ -174,7 +201,7 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
         C.steps += 1;
     else
         C.lost += 1;
-    if (P.end_state) {
+    if (!ADJ && P.end_state) {
         double *es = P.end_state + L.slot * KP_ES_STRIDE;
         double g[3];
         Pol::gpos_d(P, L.cur, g);
@@ -198,26 +225,42 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
 }
 
 // Start the peel-off ray toward observer L.o from the saved main position.
-template <class Pol> __device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L)
+template <class Pol, bool ADJ>
+__device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L, bool restart = false)
 {
     using real = typename Pol::real;
     real no[3], ev[3];
     load3(P.no[L.o], no);
     load3(P.ev[L.o], ev);
-    if (L.ev_kind == EV_EMIT) {
-        L.Pi = L.I;
-        L.Pq = 0;
-        L.Pu = 0;
-    } else {
-        real e2[3];
-        if constexpr (sizeof(real) == 8) {
-            f64::cross3(L.sd, L.e1, e2);
-            f64::peel_stokes(P.model, L.gg, P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no, ev, L.Pi,
-                             L.Pq, L.Pu);
+    if (!restart) {
+        if (L.ev_kind == EV_EMIT) {
+            L.Pi = L.I;
+            L.Pq = 0;
+            L.Pu = 0;
         } else {
-            f32::cross3(L.sd, L.e1, e2);
-            f32::peel_stokes(P.model, L.gg, (float)P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no, ev,
-                             L.Pi, L.Pq, L.Pu);
+            real e2[3];
+            if constexpr (sizeof(real) == 8) {
+                f64::cross3(L.sd, L.e1, e2);
+                f64::peel_stokes(P.model, L.gg, P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no, ev,
+                                 L.Pi, L.Pq, L.Pu);
+            } else {
+                f32::cross3(L.sd, L.e1, e2);
+                f32::peel_stokes(P.model, L.gg, (float)P.pmax, L.sd, L.e1, e2, L.I, L.Q, L.U, no,
+                                 ev, L.Pi, L.Pq, L.Pu);
+            }
+        }
+        if constexpr (ADJ) {
+            if (L.pass == 2) {
+                const uint64_t lane_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                if (L.kslot < P.adj_cap) {
+                    L.Acur = P.adj_rows[(uint64_t)L.kslot * P.adj_lanes + lane_id];
+                    L.peel_sub = 1;
+                } else {
+                    L.peel_sub = 0; // beyond the row: measure A_k first
+                }
+            } else {
+                L.peel_sub = 0;
+            }
         }
     }
 #pragma unroll
@@ -229,6 +272,7 @@ template <class Pol> __device__ __forceinline__ void start_peel(const kparams &P
     L.cur.ib = L.sib;
     Pol::relocate(P, L.cur);
     L.tau = 0.0;
+    L.kgc = -1;
     L.mode = M_PEEL;
 }
 
@@ -246,15 +290,21 @@ template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
 
 // Scatter, new free flight, Russian roulette (after the interaction's
 // peel-offs).  Leaves the lane in MAIN, or pending PA_FINISH when killed.
-template <class Pol> __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
+template <class Pol, bool ADJ>
+__device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
     real e2[3];
     bool killed = false;
+    const bool hg = P.model == KP_HG_RAYLEIGH && (P.bands[L.band].flags & KP_BAND_CELL_G);
     if constexpr (sizeof(real) == 8) {
         f64::cross3(L.cur.d, L.e1, e2);
         const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(L.u[0]))
                                                   : kp_hg_mu(L.gg, kp_u01(L.u[0]));
+        if constexpr (ADJ) {
+            if (L.pass == 2 && hg && P.g_g && L.B != 0.0)
+                atomic_add_f64(g_base(P, P.g_g) + L.gc, L.B * dlogp_hg(L.gg, mu));
+        }
         f64::scatter_stokes(P.model, P.pmax, mu, kp_u01(L.u[1]), L.cur.d, L.e1, e2, L.I, L.Q,
                             L.U);
         L.tau = -kp_log(kp_u01(L.u[2]));
@@ -271,6 +321,11 @@ template <class Pol> __device__ __forceinline__ void scatter_stage(const kparams
         f32::cross3(L.cur.d, L.e1, e2);
         const float xi0 = f32::u01(L.u[0]);
         const float mu = (P.model == KP_THOMSON) ? f32::thomson_mu(xi0) : f32::hg_mu(L.gg, xi0);
+        if constexpr (ADJ) {
+            if (L.pass == 2 && hg && P.g_g && L.B != 0.0)
+                atomic_add_f64(g_base(P, P.g_g) + L.gc,
+                               L.B * dlogp_hg((double)L.gg, (double)mu));
+        }
         f32::scatter_stokes(P.model, (float)P.pmax, mu, f32::u01(L.u[1]), L.cur.d, L.e1, e2, L.I,
                             L.Q, L.U);
         Pol::set_dir(P, L.cur, L.cur.d);
@@ -286,6 +341,7 @@ template <class Pol> __device__ __forceinline__ void scatter_stage(const kparams
             }
         }
     }
+    L.kgc = -1;
     if (killed) {
         L.pa = PA_FINISH;
         L.status = KP_END_RR;
@@ -295,12 +351,77 @@ template <class Pol> __device__ __forceinline__ void scatter_stage(const kparams
     }
 }
 
-// End of a peel-off ray: deposit, then the next observer, or resume the
-// main path (EMIT: march on; SCATTER: pending scatter).
-template <class Pol> __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
+// A_k of the finished peel-off ray: dL/dimage at its pixel . S exp(-tau)/(4 pi)
+template <class Pol> __device__ __forceinline__ double peel_A(const kparams &P, const lane_t<Pol> &L)
 {
     using real = typename Pol::real;
-    if (P.image) {
+    long px;
+    double Si, Sq, Su;
+    if constexpr (sizeof(real) == 8) {
+        const double att = kp_exp(-L.tau) * KP_INV_4PI;
+        px = f64::pixel(P, L.band, L.o, L.sx);
+        Si = L.Pi * att;
+        Sq = L.Pq * att;
+        Su = L.Pu * att;
+    } else {
+        f32::ray_t tmp;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            tmp.x[a] = L.sx[a];
+        tmp.ib = L.sib;
+        float gx[3];
+        f32::global_pos(P, tmp, gx);
+        const float att = f32::fast_exp(-(float)L.tau) * f32::kInv4Pi;
+        px = f32::pixel(P, L.band, L.o, gx);
+        Si = (double)(L.Pi * att);
+        Sq = (double)(L.Pq * att);
+        Su = (double)(L.Pu * att);
+    }
+    if (px < 0)
+        return 0.0;
+    const double *w = P.adj_w + 3 * px;
+    return w[0] * Si + w[1] * Sq + w[2] * Su;
+}
+
+// End of a peel-off ray: deposit (forward) or A_k bookkeeping (adjoint),
+// then the next observer, or resume the main path (EMIT: march on;
+// SCATTER: pending scatter).
+template <class Pol, bool ADJ>
+__device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
+{
+    using real = typename Pol::real;
+    if constexpr (ADJ) {
+        const uint64_t lane_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (L.pass == 1) {
+            const double A = peel_A<Pol>(P, L);
+            if (L.kslot < P.adj_cap)
+                P.adj_rows[(uint64_t)L.kslot * P.adj_lanes + lane_id] = A;
+            L.Atot += A;
+        } else {
+            if (L.peel_sub == 0) {
+                // measured an A_k that did not fit the row: march it again
+                L.Acur = peel_A<Pol>(P, L);
+                L.peel_sub = 1;
+                start_peel<Pol, ADJ>(P, L, true);
+                L.pa = PA_NONE;
+                return;
+            }
+            if (L.ev_kind == EV_SCATTER && P.model == KP_HG_RAYLEIGH &&
+                (P.bands[L.band].flags & KP_BAND_CELL_G) && P.g_g && L.Acur != 0.0) {
+                real no[3];
+                load3(P.no[L.o], no);
+                double mo;
+                if constexpr (sizeof(real) == 8)
+                    mo = f64::dot3(L.sd, no);
+                else
+                    mo = (double)f32::dot3(L.sd, no);
+                mo = fmin(fmax(mo, -1.0), 1.0);
+                atomic_add_f64(g_base(P, P.g_g) + L.gc, L.Acur * dlogp_hg((double)L.gg, mo));
+            }
+            L.B -= L.Acur;
+        }
+        L.kslot += 1;
+    } else if (P.image) {
         if constexpr (sizeof(real) == 8) {
             const double att = kp_exp(-L.tau) * KP_INV_4PI;
             f64::deposit(P, L.band, L.o, L.sx, L.Pi * att, L.Pq * att, L.Pu * att);
@@ -318,7 +439,7 @@ template <class Pol> __device__ __forceinline__ void peel_end_stage(const kparam
     }
     L.o += 1;
     if (L.o < P.n_incl) {
-        start_peel<Pol>(P, L);
+        start_peel<Pol, ADJ>(P, L);
         L.pa = PA_NONE;
         return;
     }
@@ -331,6 +452,7 @@ template <class Pol> __device__ __forceinline__ void peel_end_stage(const kparam
     L.cur.ib = L.sib;
     Pol::relocate(P, L.cur);
     L.tau = L.tau_main;
+    L.kgc = -1;
     if (L.ev_kind == EV_EMIT) {
         L.mode = M_MAIN;
         L.pa = PA_NONE;
@@ -339,7 +461,8 @@ template <class Pol> __device__ __forceinline__ void peel_end_stage(const kparam
     }
 }
 
-template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
+template <class Pol, bool ADJ>
+__device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
 {
     L.band = band_of(P, L.slot);
     L.pid = P.packet_offset + (L.slot - P.band_start[L.band]);
@@ -353,16 +476,24 @@ template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &
     L.Q = 0;
     L.U = 0;
     Pol::at_source(P, L.cur);
+    L.kgc = -1;
     L.events = 0;
     L.steps = P.step_max;
     L.ncross = 0;
     L.npeel = 0;
     L.pa = PA_NONE;
+    if constexpr (ADJ) {
+        L.kslot = 0;
+        if (L.pass == 1)
+            L.Atot = 0.0;
+        else
+            L.B = L.Atot;
+    }
     if (P.n_incl > 0) {
         save_main(L);
         L.ev_kind = EV_EMIT;
         L.o = 0;
-        start_peel<Pol>(P, L);
+        start_peel<Pol, ADJ>(P, L);
     } else {
         L.mode = M_MAIN;
     }
@@ -370,7 +501,7 @@ template <class Pol> __device__ __forceinline__ void emit_packet(const kparams &
 
 // Interaction at the current point: Philox words of the event, absorption,
 // then the peel-offs (or the scatter directly when there is no observer).
-template <class Pol>
+template <class Pol, bool ADJ>
 __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
@@ -397,19 +528,29 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
         L.U *= alb;
     }
     L.gg = gg;
+    L.alb = alb;
+    if constexpr (ADJ) {
+        if (L.pass == 2 && L.B != 0.0) {
+            if (P.g_logrho)
+                atomic_add_f64(g_base(P, P.g_logrho) + gc, L.B);
+            if ((bd.flags & KP_BAND_CELL_ALBEDO) && P.g_alb)
+                atomic_add_f64(g_base(P, P.g_alb) + gc, L.B / (double)alb);
+        }
+    }
     if (P.n_incl > 0) {
         save_main(L);
         L.ev_kind = EV_SCATTER;
         L.o = 0;
-        start_peel<Pol>(P, L);
+        start_peel<Pol, ADJ>(P, L);
         L.pa = PA_NONE;
     } else {
         L.pa = PA_SCATTER;
     }
 }
 
-template <class Pol>
-__device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C)
+template <class Pol, bool ADJ>
+__device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
+                             double &jsum)
 {
     using real = typename Pol::real;
     lane_t<Pol> L;
@@ -417,6 +558,12 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     L.pa = PA_NONE;
     L.kgc = -1;
     L.kcache = 0.0f;
+    L.pass = 1;
+    L.jsum = 0.0;
+    L.Atot = 0.0;
+    L.B = 0.0;
+    L.Acur = 0.0;
+    L.peel_sub = 0;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
@@ -431,13 +578,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
             // ---------------- action phase: every pending lane together
             if (L.mode == M_PEND && L.pa == PA_PEEL_END)
-                peel_end_stage<Pol>(P, L);
+                peel_end_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_INTERACT)
-                interact_stage<Pol>(P, L);
+                interact_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_SCATTER)
-                scatter_stage<Pol>(P, L);
+                scatter_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
-                finish(P, L, C, L.status);
+                finish<Pol, ADJ>(P, L, C, L.status);
             // refill idle lanes from the wave's slot range
             const uint64_t need = __ballot(L.mode == M_NEED);
             if (need) {
@@ -462,7 +609,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 if (L.mode == M_NEED) {
                     if (my < P.total) {
                         L.slot = my;
-                        emit_packet<Pol>(P, L);
+                        emit_packet<Pol, ADJ>(P, L);
                     } else {
                         L.mode = M_DONE;
                     }
@@ -502,8 +649,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     double li = L.tau / kap;
                     if (li > l)
                         li = l;
-                    if (P.J)
-                        atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (li / V));
+                    if constexpr (ADJ) {
+                        if (L.pass == 2 && P.g_logrho && L.B != 0.0)
+                            atomic_add_f64(g_base(P, P.g_logrho) + gc, -((kap * li) * L.B));
+                    } else {
+                        if (P.J)
+                            atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (li / V));
+                    }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
                     L.mode = M_PEND;
@@ -511,13 +663,24 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     continue;
                 }
                 L.tau = L.tau - dtau;
-                if (P.J)
-                    atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (l / V));
+                if constexpr (ADJ) {
+                    if (L.pass == 2 && P.g_logrho && L.B != 0.0)
+                        atomic_add_f64(g_base(P, P.g_logrho) + gc, -(dtau * L.B));
+                } else {
+                    if (P.J)
+                        atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (l / V));
+                }
             } else {
                 const float iV = f32::fast_rcp(P.vol32[L.cur.ib]);
                 if (kap > 0.0f && (double)dtau >= L.tau) {
                     const float li = fminf((float)L.tau * f32::fast_rcp(kap), l);
-                    L.acc.add(P, L.band, gc, L.I * li * iV);
+                    if constexpr (ADJ) {
+                        if (L.pass == 2 && P.g_logrho && L.B != 0.0)
+                            atomic_add_f64(g_base(P, P.g_logrho) + gc,
+                                           -((double)(kap * li) * L.B));
+                    } else {
+                        L.acc.add(P, L.band, gc, L.I * li * iV);
+                    }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
                     L.mode = M_PEND;
@@ -525,11 +688,20 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     continue;
                 }
                 L.tau -= (double)dtau;
-                L.acc.add(P, L.band, gc, L.I * l * iV);
+                if constexpr (ADJ) {
+                    if (L.pass == 2 && P.g_logrho && L.B != 0.0)
+                        atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.B));
+                } else {
+                    L.acc.add(P, L.band, gc, L.I * l * iV);
+                }
             }
         } else {
             L.tau = L.tau + (double)dtau;
             L.npeel += 1;
+            if constexpr (ADJ) {
+                if (L.pass == 2 && L.peel_sub == 1 && P.g_logrho && L.Acur != 0.0)
+                    atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.Acur));
+            }
             if (L.tau > kTauCut) {
                 L.mode = M_PEND;
                 L.pa = PA_PEEL_END;
@@ -551,6 +723,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             }
         }
     }
+    jsum = L.jsum;
 }
 
 } // namespace kp

@@ -194,11 +194,9 @@ __device__ __forceinline__ void global_pos(const kparams &P, const ray_t &r, flo
         gx[a] = fmaf(r.x[a], P.h32[3 * r.ib + a], P.org32[3 * r.ib + a]);
 }
 
-__device__ __forceinline__ void deposit(const kparams &P, int band, int o, const float *gx,
-                                        float I, float Q, float U)
+// Image pixel (linear index into [band][incl][N][N]) of global point gx, or -1.
+__device__ __forceinline__ long pixel(const kparams &P, int band, int o, const float *gx)
 {
-    if (!P.image)
-        return;
     const int N = P.image_n;
     float rel[3];
 #pragma unroll
@@ -212,8 +210,18 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     const float fx = floorf((uh + half) * sc);
     const float fy = floorf((uv + half) * sc);
     if (fx < 0.0f || fy < 0.0f || fx >= (float)N || fy >= (float)N)
+        return -1;
+    return (long)((((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
+}
+
+__device__ __forceinline__ void deposit(const kparams &P, int band, int o, const float *gx,
+                                        float I, float Q, float U)
+{
+    if (!P.image)
         return;
-    const size_t pix = (((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
+    const long pix = pixel(P, band, o, gx);
+    if (pix < 0)
+        return;
     atomic_add_f64(img_base(P) + 3 * pix + 0, (double)I);
     atomic_add_f64(img_base(P) + 3 * pix + 1, (double)Q);
     atomic_add_f64(img_base(P) + 3 * pix + 2, (double)U);

@@ -195,11 +195,9 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const double 
     return tau;
 }
 
-__device__ __forceinline__ void deposit(const kparams &P, int band, int o, const double *x,
-                                        double I, double Q, double U)
+// Image pixel (linear index into [band][incl][N][N]) of point x, or -1.
+__device__ __forceinline__ long pixel(const kparams &P, int band, int o, const double *x)
 {
-    if (!P.image)
-        return;
     const int N = P.image_n;
     double rel[3];
 #pragma unroll
@@ -210,8 +208,18 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     const double fx = floor((uh + P.image_half) * P.pix_scale);
     const double fy = floor((uv + P.image_half) * P.pix_scale);
     if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+        return -1;
+    return (long)((((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
+}
+
+__device__ __forceinline__ void deposit(const kparams &P, int band, int o, const double *x,
+                                        double I, double Q, double U)
+{
+    if (!P.image)
         return;
-    const size_t pix = (((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
+    const long pix = pixel(P, band, o, x);
+    if (pix < 0)
+        return;
     atomic_add_f64(img_base(P) + 3 * pix + 0, I);
     atomic_add_f64(img_base(P) + 3 * pix + 1, Q);
     atomic_add_f64(img_base(P) + 3 * pix + 2, U);
