@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--skip-f64", action="store_true")
     p.add_argument("--skip-adjoint", action="store_true")
     p.add_argument("--adjoint-packets", type=float, default=1e7)
+    p.add_argument("--skip-batched", action="store_true")
+    p.add_argument("--batched-scale", type=float, default=1.0,
+                   help="scale of the per-band packet counts of the 64-band workload")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 512 MiB device-to-device copy (3 times) so "
                         "rocprofv3 counters can be scaled by a kernel of known traffic")
@@ -126,6 +129,65 @@ def adjoint_ratio(n):
                                 for k, v in gr64.items() if k != "jlin"},
             "grad_norms_mixed": {k: float(torch.linalg.vector_norm(v))
                                  for k, v in gr32.items() if k != "jlin"}}
+
+
+def batched(steps, scale):
+    """64-band batched forward (config 3): autotuned schedule, all bands in one
+    persistent launch captured into a hipGraph; mixed precision vs the FP64
+    path on the same packets."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi, configs
+    from kratos_polrad_amd.engine import Engine, RunSpec
+
+    w = configs.config3(precision=_abi.KP_F32, scale_packets=scale)
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    res = {"workload": w.name, "bands": len(w.run.bands), "packets_per_step": w.run.total_packets,
+           "min_band_packets": min(b.n_packets for b in w.run.bands),
+           "max_band_packets": max(b.n_packets for b in w.run.bands)}
+    t0 = time.perf_counter()
+    res["schedule"] = eng.autotune(w.source, w.run, sample_packets=2_000_000)
+    res["autotune_s"] = time.perf_counter() - t0
+
+    def run_graph(prec, offset=0):
+        run = RunSpec(**{**w.run.__dict__, "precision": prec, "packet_offset": offset})
+        out = eng.alloc_outputs(run)
+        g = eng.graph(w.source, run, out, n_chunks=1)
+        g.launch()  # warm-up replay
+        torch.cuda.synchronize()
+        for v in out.values():
+            v.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cnt = out["counters"].cpu().numpy().astype(np.float64)
+        img = out["image"].cpu().numpy()[:, 0]
+        del g
+        return wall, cnt, img
+
+    wall32, cnt32, img32 = run_graph(_abi.KP_F32)
+    res["value"] = float((cnt32[0] + cnt32[1]) / wall32)
+    res["ms_per_step"] = 1e3 * wall32 / steps
+    wall64, cnt64, img64 = run_graph(_abi.KP_F64)
+    res["fp64_value"] = float((cnt64[0] + cnt64[1]) / wall64)
+    res["speedup_mixed_vs_fp64"] = res["value"] / res["fp64_value"]
+    # image error per band against an independent FP64 run (other packet
+    # ids), relative to the band's I norm: mixed and FP64 must show the same
+    # error (the Monte Carlo noise); median and worst band reported
+    _, _, ref = run_graph(_abi.KP_F64, offset=1 << 40)
+    e32 = np.array([[img_err(img32[b], ref[b])[k] for k in "IQU"] for b in range(len(ref))])
+    e64 = np.array([[img_err(img64[b], ref[b])[k] for k in "IQU"] for b in range(len(ref))])
+    for name, e in (("mixed_vs_independent_fp64", e32), ("fp64_vs_independent_fp64", e64)):
+        res["image_error_" + name] = {
+            "median_band": dict(zip("IQU", np.median(e, axis=0).tolist())),
+            "worst_band": dict(zip("IQU", e.max(axis=0).tolist()))}
+    res["image_error_ratio_mixed_over_fp64_median"] = float(np.median(e32[:, 0] / e64[:, 0]))
+    return res
 
 
 def main():
@@ -228,6 +290,9 @@ def main():
             "mixed_vs_independent_fp64": img_err(img32, ref),
             "fp64_vs_independent_fp64": img_err(img64, ref),
         }
+
+    if not args.skip_batched:
+        res["batched"] = batched(args.steps, args.batched_scale)
 
     if not args.skip_adjoint:
         res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))

@@ -228,6 +228,33 @@ kp_status kp_get_schedule(const kp_engine *eng, kp_schedule *sched);
 kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,
                      kp_tallies *out, void *stream);
 
+/* Autotuner: times candidate schedules (action-phase batch, register budget,
+ * replica counts, blocks per CU, block size; coordinate descent from the
+ * current schedule) on a sample of about sample_packets packets of `run`
+ * with HIP events on `stream`, installs the fastest and returns it.  Results
+ * are cached per workload shape (precision, bands, observers, image size,
+ * scattering model, mesh size, packet-count octave); force != 0 re-times.
+ * Blocking call.  No reference counterpart (the reference takes n_th from
+ * the input file, driver_base.h:30). */
+kp_status kp_autotune(kp_engine *eng, const kp_source *src, const kp_run *run,
+                      uint64_t sample_packets, int32_t force, kp_schedule *best,
+                      void *stream);
+
+/* hipGraph of a forward run: n_chunks chunk launches (chunk k draws packet
+ * ids packet_offset + k*max_b(n_packets_b)) and the accumulator folds,
+ * captured once with the engine's current schedule; kp_graph_launch replays
+ * the whole sequence with one call and ADDS into the tallies given at
+ * capture (same packet ids every replay).  The graph keeps its own band
+ * table and launch scratch, so eager calls on the engine do not disturb it;
+ * it reads the engine's medium, which kp_set_media rewrites in place.
+ * end_state requires n_chunks == 1.  The engine must outlive its graphs.
+ * Replaces the per-cycle host loop of the reference (driver.h:59-79). */
+typedef struct kp_graph kp_graph;
+kp_status kp_graph_create(kp_engine *eng, const kp_source *src, const kp_run *run,
+                          kp_tallies *out, int32_t n_chunks, kp_graph **graph);
+kp_status kp_graph_launch(kp_graph *graph, void *stream);
+void kp_graph_destroy(kp_graph *graph);
+
 /* Device gradient outputs of kp_adjoint, per cell (each may be NULL);
  * kp_adjoint ADDS into them.  jlin (device scalar) receives the primal
  * value sum(dL_dimage * image) of the replayed histories. */

@@ -136,6 +136,12 @@ SIGNATURES = {
     "kp_forward": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.POINTER(Tallies), _P]),
     "kp_adjoint": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), _P, C.POINTER(Grads), _P]),
     "kp_set_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
+    "kp_autotune": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.c_uint64, C.c_int32,
+                              C.POINTER(Schedule), _P]),
+    "kp_graph_create": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.POINTER(Tallies),
+                                  C.c_int32, C.POINTER(_P)]),
+    "kp_graph_launch": (C.c_int, [_P, _P]),
+    "kp_graph_destroy": (None, [_P]),
     "kp_get_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
 }
 

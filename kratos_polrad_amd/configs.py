@@ -11,6 +11,11 @@ seeded Gaussian random field with a k^-11/3 power spectrum), refinement of
 every block whose max |grad rho| dx / rho exceeds 0.5, HG g=0.6, albedo 0.5,
 Rayleigh-form linear polarization with p_max 0.5.
 
+Config 3 (batched): the config-2 mesh and medium seen at 64 wavelengths
+log-spaced over 0.3-3 um: extinction scaled by (lambda/0.55 um)^-1.7, HG g
+falling linearly from 0.7 to 0.3 across the bands, albedo 0.5, uneven packet
+counts drawn log-uniform in [1e5, 1e7] from the seed; all bands in one launch.
+
 Config 4 (adjoint): the config-2 field on a 2-level mesh of 3^3 root blocks
 (96^3 effective); the "observed" Q/U image is rendered from the same field
 times an extra seeded lognormal factor (sigma 0.2); loss = sum of squared Q
@@ -172,6 +177,34 @@ def config2(n_packets: int = 100_000_000, precision: int = _abi.KP_F32, seed: in
                   scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
                   incl_deg=(90.0,), image_n=128, image_half=0.5)
     return Workload("config2_amr3_r2_lognormal_hg", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
+
+
+def config3_bands(n_bands: int = 64, seed: int = 42, n_min: float = 1e5, n_max: float = 1e7,
+                  lam_min: float = 0.3, lam_max: float = 3.0, lam_ref: float = 0.55,
+                  slope: float = -1.7, g_first: float = 0.7, g_last: float = 0.3,
+                  albedo: float = 0.5, scale_packets: float = 1.0):
+    """Per-wavelength bands: (wavelengths um, [BandSpec])."""
+    lam = lam_min * (lam_max / lam_min) ** (np.arange(n_bands) / max(n_bands - 1, 1))
+    g = g_first + (g_last - g_first) * np.arange(n_bands) / max(n_bands - 1, 1)
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_bands)) * scale_packets
+    bands = [BandSpec(n_packets=max(1, int(round(n[k]))), kappa_scale=float((lam[k] / lam_ref) ** slope),
+                      albedo=albedo, g=float(g[k])) for k in range(n_bands)]
+    return lam, bands
+
+
+def config3(precision: int = _abi.KP_F32, seed: int = 42, tau_centre: float = 5.0,
+            scale_packets: float = 1.0, n_bands: int = 64, **kw) -> Workload:
+    """64-band batched forward on the config-2 mesh (kappa_rho of config 2 is
+    the 0.55 um extinction)."""
+    mesh, rho, rho_at = amr_density(**kw)
+    kappa = (rho * (tau_centre / centre_tau_norm(rho_at))).astype(np.float32)
+    albedo = np.full(mesh.n_cells, 0.5, dtype=np.float32)
+    g = np.full(mesh.n_cells, 0.6, dtype=np.float32)
+    _, bands = config3_bands(n_bands=n_bands, seed=seed, scale_packets=scale_packets)
+    run = RunSpec(bands=bands, seed=seed, scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5,
+                  precision=precision, incl_deg=(90.0,), image_n=128, image_half=0.5)
+    return Workload("config3_amr3_64band_batched", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
 
 
 def lognormal_factor(x: np.ndarray, sigma: float, seed: int, field_n: int = 112) -> np.ndarray:

@@ -53,6 +53,8 @@ struct kparams {
     double w0;
     int src_ib;
     int src_i[3];
+    int64_t src_gc;  // global index of the source cell
+    int lds_n;       // doubles of LDS accumulators per workgroup (0: none)
     float src_xl[3]; // source position in block-local cell units (FP32 path)
     // run
     uint32_t seed;
@@ -80,6 +82,7 @@ struct kparams {
     // replica b % R, spreading FP64 atomics on hot cells/pixels
     uint32_t j_nrep, img_nrep;
     uint64_t j_stride, img_stride; // elements per replica
+    uint64_t j_cs, j_bs;           // J element of (cell gc, band b): gc*j_cs + b*j_bs
     // adjoint (path replay) launches
     const double *adj_w;           // dL/dimage, image layout
     double *g_logrho, *g_alb, *g_g; // gradient replica bases (may be null)
@@ -100,6 +103,11 @@ __device__ __forceinline__ double *j_base(const kparams &P)
 {
     return P.J + (uint64_t)(blockIdx.x % P.j_nrep) * P.j_stride;
 }
+// address of the J tally of global cell gc in band b (this workgroup's replica)
+__device__ __forceinline__ double *j_at(const kparams &P, uint64_t gc, int b)
+{
+    return j_base(P) + gc * P.j_cs + (uint64_t)b * P.j_bs;
+}
 __device__ __forceinline__ double *img_base(const kparams &P)
 {
     return P.image + (uint64_t)(blockIdx.x % P.img_nrep) * P.img_stride;
@@ -110,13 +118,33 @@ __device__ __forceinline__ void atomic_add_f64(double *p, double v)
     unsafeAtomicAdd(p, v); // native global_atomic_add_f64 (coarse-grained memory)
 }
 
-// Band of a global packet slot (band_start is small and cache resident).
+// Band of a global packet slot: the last b with band_start[b] <= slot
+// (binary search; band_start is small and cache resident).  Empty bands
+// (equal starts) are skipped the same way a linear scan would skip them.
 __device__ __forceinline__ int band_of(const kparams &P, uint64_t slot)
 {
-    int b = 0;
-    while (b + 1 < P.n_bands && slot >= P.band_start[b + 1])
-        ++b;
-    return b;
+    int lo = 0, hi = P.n_bands - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slot >= P.band_start[mid])
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// J deposit of v into cell gc of band b.  `sh` (persistent launches) is the
+// workgroup's LDS accumulator block: its first n_bands doubles collect the
+// source cell's J -- every packet starts in that cell, so those deposits
+// would otherwise serialise on one global address.
+__device__ __forceinline__ void j_deposit(const kparams &P, double *sh, int64_t gc, int b,
+                                          double v)
+{
+    if (sh && gc == P.src_gc)
+        atomicAdd(sh + b, v);
+    else
+        atomic_add_f64(j_at(P, (uint64_t)gc, b), v);
 }
 
 } // namespace kp

@@ -10,7 +10,10 @@
 // rad_t::flx (radiation.h:24-35).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
+#include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -25,6 +28,16 @@
         if (e_ != hipSuccess)                                                               \
             throw kp::error(KP_ERR_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
     } while (0)
+
+// Device scratch of forward launches: the persistent slot counter and the
+// tally / image replicas (zero between launches).  The engine owns one set;
+// each captured graph owns its own, so replays never share scratch with
+// eager launches.
+struct fwd_bufs {
+    unsigned long long *work = nullptr;
+    double *j_rep = nullptr, *img_rep = nullptr;
+    size_t j_cap = 0, img_cap = 0; // elements
+};
 
 struct kp_engine {
     int device = 0;
@@ -50,15 +63,25 @@ struct kp_engine {
     // launch shape
     kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0};  // last: waves_per_simd
     int n_cu = 0;
-    unsigned long long *work = nullptr; // persistent slot counter
-    // tally / image replicas (zero between launches)
-    double *j_rep = nullptr, *img_rep = nullptr;
-    size_t j_rep_cap = 0, img_rep_cap = 0; // elements
+    fwd_bufs fb;                        // forward launch scratch
     // adjoint: per-thread rows of peel-off contributions, gradient replicas
     double *adj_scratch = nullptr;
     size_t adj_cap = 0;
     double *g_rep = nullptr;
     size_t g_rep_cap = 0;
+    std::map<std::string, kp_schedule> tuned; // autotuner cache
+};
+
+// A captured forward run.  Holds its own copy of the band table and its own
+// launch scratch; reads the engine's block table and medium (kp_set_media
+// rewrites those in place, so a replay sees the current medium).
+struct kp_graph {
+    kp_engine *e = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    kp::dband *bands = nullptr;
+    uint64_t *band_start = nullptr;
+    fwd_bufs fb;
 };
 
 namespace {
@@ -170,6 +193,46 @@ __global__ void __launch_bounds__(256) k_fold(double *out, double *rep, uint64_t
     }
 }
 
+// Band-major replicas folded into the caller's cell-major J:
+// out[c*nb + b] += sum_r rep[r][b*nc + c]; the replicas are left zero.
+// 64-cell x 16-band tiles transposed through LDS so that both the replica
+// reads (along cells) and the output updates (along bands) are contiguous.
+__global__ void __launch_bounds__(256) k_fold_t(double *out, double *rep, uint64_t nc, int nb,
+                                                uint32_t R)
+{
+    __shared__ double tile[16][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const uint64_t n_ct = (nc + 63) / 64;
+    const uint64_t n_bt = (uint64_t)(nb + 15) / 16;
+    const uint64_t nrep = nc * (uint64_t)nb;
+    for (uint64_t t = blockIdx.x; t < n_ct * n_bt; t += gridDim.x) {
+        const uint64_t c0 = (t % n_ct) * 64;
+        const int b0 = (int)(t / n_ct) * 16;
+        for (int k = ty; k < 16; k += 4) {
+            const int b = b0 + k;
+            const uint64_t c = c0 + tx;
+            double sum = 0.0;
+            if (b < nb && c < nc) {
+                for (uint32_t r = 0; r < R; ++r) {
+                    double *q = rep + r * nrep + (uint64_t)b * nc + c;
+                    sum += *q;
+                    *q = 0.0;
+                }
+            }
+            tile[k][tx] = sum;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < 1024; q += 256) {
+            const int bb = q & 15, cc = q >> 4;
+            const int b = b0 + bb;
+            const uint64_t c = c0 + cc;
+            if (b < nb && c < nc)
+                out[c * nb + b] += tile[bb][cc];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -180,10 +243,26 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
 template <class Pol, int WPS, bool ADJ>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
+    extern __shared__ double k_sh[];
+    double *sh = nullptr;
+    if constexpr (!ADJ) {
+        if (P.lds_n > 0) {
+            for (int k = threadIdx.x; k < P.lds_n; k += blockDim.x)
+                k_sh[k] = 0.0;
+            __syncthreads();
+            sh = k_sh;
+        }
+    }
     kp::lane_counters C;
     double jsum = 0.0;
-    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum);
+    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum, sh);
     flush_counters(P, C);
+    if constexpr (!ADJ) {
+        if (P.lds_n > 0) {
+            __syncthreads();
+            kp::flush_lds_acc<Pol>(P, k_sh);
+        }
+    }
     if constexpr (ADJ) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
@@ -228,6 +307,14 @@ __global__ void __launch_bounds__(256) k_adjoint(kp::kparams P, kp::adj::aparams
         atomicAdd(Q.jlin, jsum);
 }
 
+void free_bufs(fwd_bufs &b)
+{
+    dfree(b.work);
+    dfree(b.j_rep);
+    dfree(b.img_rep);
+    b.j_cap = b.img_cap = 0;
+}
+
 void free_all(kp_engine *e)
 {
     dfree(e->xf0);
@@ -245,14 +332,11 @@ void free_all(kp_engine *e)
     dfree(e->g);
     dfree(e->bands_d);
     dfree(e->band_start_d);
-    dfree(e->work);
-    dfree(e->j_rep);
-    dfree(e->img_rep);
+    free_bufs(e->fb);
     dfree(e->adj_scratch);
     e->adj_cap = 0;
     dfree(e->g_rep);
     e->g_rep_cap = 0;
-    e->j_rep_cap = e->img_rep_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
     e->stage = nullptr;
@@ -325,11 +409,20 @@ void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t 
 
 // Replica count for an array of n elements: the schedule's value, or as many
 // as fit a byte budget (capped by the number of resident workgroups).  The
-// default budgets keep the replicas inside the 256 MB Infinity Cache: replicas
-// that spill to HBM turn every FP64 atomic into a line fetch + write-back.
-uint32_t pick_replicas(int requested, uint64_t n, uint64_t blocks, uint64_t budget_bytes)
+// default budgets keep the live part of the replicas inside the 256 MB
+// Infinity Cache: replicas that spill to HBM turn every FP64 atomic into a
+// line fetch + write-back.  n_live is the part of one replica that the
+// packets in flight update (one band's slice when bands are laid out
+// band-major: slots are claimed in band order); the whole set of replicas
+// is capped at 4 GiB.
+uint32_t pick_replicas(int requested, uint64_t n, uint64_t n_live, uint64_t blocks,
+                       uint64_t budget_bytes)
 {
-    uint64_t R = requested > 0 ? (uint64_t)requested : budget_bytes / (n * sizeof(double) + 1);
+    uint64_t R =
+        requested > 0 ? (uint64_t)requested : budget_bytes / (n_live * sizeof(double) + 1);
+    const uint64_t hard = (4ull << 30) / (n * sizeof(double) + 1);
+    if (R > hard)
+        R = hard;
     if (R > blocks)
         R = blocks;
     if (R > 256)
@@ -383,6 +476,8 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
         P.bands = e->bands_d;
         P.band_start = e->band_start_d;
         P.n_bands = run->n_bands;
+        P.j_cs = (uint64_t)run->n_bands; // cell-major J unless a plan says otherwise
+        P.j_bs = 1;
         P.total = total;
         P.packet_offset = run->packet_offset;
         int ib, ii[3];
@@ -394,6 +489,8 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
             P.src_xl[a] = (float)((src->x[a] - e->xf0_h[3 * ib + a]) / e->dx0_h[3 * ib + a]);
         }
         P.src_ib = ib;
+        P.lds_n = 0;
+        P.src_gc = (int64_t)ib * e->n3 + ii[0] + (int64_t)e->n_cell * (ii[1] + (int64_t)e->n_cell * ii[2]);
         P.w0 = src->weight;
         P.seed = run->seed;
         P.model = run->scatter_model;
@@ -418,6 +515,147 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
         P.image_n = run->image_n;
         P.image_half = run->image_half;
         P.pix_scale = run->image_n > 0 ? (double)run->image_n / (2.0 * run->image_half) : 0.0;
+}
+
+// Launch plan of a forward run: grid and replica counts; allocates replica
+// buffers (never called inside a stream capture).
+// bytes of LDS accumulators a persistent forward workgroup may take
+constexpr uint64_t kMaxLdsAcc = 32768;
+
+struct fwd_plan {
+    uint64_t blocks = 0;
+    int T = 256;
+    uint64_t nJ = 0, nI = 0;
+    bool j_band_major = false; // J accumulated band-major in replicas, folded by k_fold_t
+};
+
+fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total,
+                      const kp_run *run, hipStream_t stream)
+{
+    const kp_schedule &S = e->sched;
+    fwd_plan pl;
+    if (S.kind == KP_SCHED_HISTORY) {
+        pl.T = 256;
+        pl.blocks = (total + pl.T - 1) / pl.T;
+        if (pl.blocks > 0x7fffffffull)
+            pl.blocks = 0x7fffffffull;
+    } else {
+        pl.T = S.block_threads;
+        pl.blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
+        const uint64_t need = (total + pl.T - 1) / pl.T;
+        if (pl.blocks > need)
+            pl.blocks = need;
+    }
+    pl.nJ = (uint64_t)e->n_blocks * e->n3 * run->n_bands;
+    pl.nI = (uint64_t)run->n_bands * run->n_incl * run->image_n * run->image_n * 3;
+    P.j_nrep = P.img_nrep = 1;
+    P.j_stride = pl.nJ;
+    P.img_stride = pl.nI;
+    const uint64_t nc = (uint64_t)e->n_blocks * e->n3;
+    P.j_cs = (uint64_t)run->n_bands;
+    P.j_bs = 1;
+    if (P.J) {
+        P.j_nrep = pick_replicas(S.j_replicas, pl.nJ, run->n_bands > 1 ? nc : pl.nJ, pl.blocks,
+                                 160ull << 20);
+        // several bands: the packets in flight belong to one or two bands
+        // (slots are claimed in order), so a band-major accumulator keeps
+        // the live part of J to one band's cells; cell-major output is made
+        // by the transposing fold
+        pl.j_band_major = run->n_bands > 1;
+        if (pl.j_band_major) {
+            P.j_cs = 1;
+            P.j_bs = nc;
+        }
+        if (P.j_nrep > 1 || pl.j_band_major)
+            ensure_replicas(fb.j_rep, fb.j_cap, (uint64_t)P.j_nrep * pl.nJ, stream);
+    }
+    if (P.image && pl.nI > 0) {
+        P.img_nrep = pick_replicas(S.img_replicas, pl.nI, pl.nI / (uint64_t)run->n_bands,
+                                   pl.blocks, 32ull << 20);
+        if (P.img_nrep > 1)
+            ensure_replicas(fb.img_rep, fb.img_cap, (uint64_t)P.img_nrep * pl.nI, stream);
+    }
+    // LDS accumulators of the source cell's J and the emission peel-offs
+    P.lds_n = 0;
+    if (S.kind == KP_SCHED_PERSISTENT) {
+        const uint64_t n = (uint64_t)run->n_bands * (1 + 3 * (uint64_t)run->n_incl);
+        if (n * sizeof(double) <= kMaxLdsAcc)
+            P.lds_n = (int)n;
+    }
+    if (!fb.work) {
+        fb.work = dmalloc<unsigned long long>(1);
+        KP_HIP(hipMemsetAsync(fb.work, 0, sizeof(unsigned long long), stream));
+    }
+    P.work = fb.work;
+    P.chunk = (uint32_t)S.chunk;
+    P.batch = (uint32_t)S.event_batch;
+    return pl;
+}
+
+// Launches of a planned forward run on `stream`: n_chunks transport kernels
+// (chunk k draws packet ids offset by k * chunk_stride), then the replica
+// folds into the caller's arrays.  Kernels only -- no memsets, no host
+// synchronisation -- so the sequence can be captured into a hipGraph.
+void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, const fwd_plan &pl,
+                   const kp_run *run, hipStream_t stream, int n_chunks, uint64_t chunk_stride)
+{
+    double *J_out = P.J, *I_out = P.image;
+    if (P.J && (P.j_nrep > 1 || pl.j_band_major))
+        P.J = fb.j_rep;
+    if (P.image && P.img_nrep > 1)
+        P.image = fb.img_rep;
+    const uint64_t off0 = P.packet_offset;
+    for (int k = 0; k < n_chunks; ++k) {
+        P.packet_offset = off0 + (uint64_t)k * chunk_stride;
+        if (S.kind == KP_SCHED_HISTORY) {
+            if (run->precision == KP_F64)
+                hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)pl.blocks), dim3(pl.T), 0,
+                                   stream, P);
+            else
+                hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)pl.blocks), dim3(pl.T), 0,
+                                   stream, P);
+        } else {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.work);
+            hipLaunchKernelGGL(persist_variant<false>(run->precision, S.waves_per_simd),
+                               dim3((unsigned)pl.blocks), dim3(pl.T),
+                               (size_t)P.lds_n * sizeof(double), stream, P);
+        }
+    }
+    if (J_out && pl.j_band_major)
+        hipLaunchKernelGGL(k_fold_t, dim3(4096), dim3(256), 0, stream, J_out, fb.j_rep,
+                           pl.nJ / (uint64_t)run->n_bands, run->n_bands, P.j_nrep);
+    else if (J_out && P.j_nrep > 1)
+        hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, stream, J_out, fb.j_rep, pl.nJ,
+                           P.j_nrep);
+    if (I_out && P.img_nrep > 1)
+        hipLaunchKernelGGL(k_fold, dim3(512), dim3(256), 0, stream, I_out, fb.img_rep, pl.nI,
+                           P.img_nrep);
+    KP_HIP(hipGetLastError());
+}
+
+// Validation shared by kp_forward and kp_graph_create.
+void check_forward_args(const kp_engine *e, const kp_source *src, const kp_run *run,
+                        const kp_tallies *out, const char *who)
+{
+    const std::string w(who);
+    if (!e || !src || !run || !out)
+        throw kp::error(KP_ERR_ARG, w + ": null argument");
+    if (!e->media_set)
+        throw kp::error(KP_ERR_STATE, w + ": kp_set_media not called");
+    if (run->n_bands < 1 || !run->bands)
+        throw kp::error(KP_ERR_ARG, w + ": need at least one band");
+    if (run->n_incl < 0 || run->n_incl > kp::kMaxIncl || (run->n_incl > 0 && !run->incl_deg))
+        throw kp::error(KP_ERR_ARG, w + ": n_incl must be in [0,8]");
+    if (run->n_incl > 0 && (e->periodic[0] || e->periodic[1] || e->periodic[2]))
+        throw kp::error(KP_ERR_UNSUPPORTED, w + ": peel-off images need a non-periodic domain");
+    if (run->n_incl > 0 && out->image && (run->image_n < 1 || !(run->image_half > 0.0)))
+        throw kp::error(KP_ERR_ARG, w + ": bad image geometry");
+    if (run->scatter_model != KP_THOMSON && run->scatter_model != KP_HG_RAYLEIGH)
+        throw kp::error(KP_ERR_ARG, w + ": unknown scatter_model");
+    if (run->precision != KP_F64 && run->precision != KP_F32)
+        throw kp::error(KP_ERR_ARG, w + ": unknown precision");
+    if (!(run->rr_survive > 0.0 && run->rr_survive <= 1.0))
+        throw kp::error(KP_ERR_ARG, w + ": rr_survive must be in (0,1]");
 }
 
 } // namespace
@@ -467,7 +705,7 @@ kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **ou
             upload(e->nb_shift, m->nb_shift, 24 * nb);
             upload(e->d_lvl, m->d_lvl, 6 * nb);
             upload(e->phybnd, m->phybnd, nb);
-            e->work = dmalloc<unsigned long long>(1);
+            e->fb.work = dmalloc<unsigned long long>(1);
             KP_HIP(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device));
             std::vector<float> h(3 * nb), o(3 * nb), v(nb);
             for (int b = 0; b < nb; ++b) {
@@ -533,28 +771,9 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
                      void *stream_)
 {
     return kp::guard([&] {
-        if (!e || !src || !run || !out)
-            throw kp::error(KP_ERR_ARG, "kp_forward: null argument");
-        if (!e->media_set)
-            throw kp::error(KP_ERR_STATE, "kp_forward: kp_set_media not called");
-        if (run->n_bands < 1 || !run->bands)
-            throw kp::error(KP_ERR_ARG, "kp_forward: need at least one band");
-        if (run->n_incl < 0 || run->n_incl > kp::kMaxIncl || (run->n_incl > 0 && !run->incl_deg))
-            throw kp::error(KP_ERR_ARG, "kp_forward: n_incl must be in [0,8]");
-        if (run->n_incl > 0 && (e->periodic[0] || e->periodic[1] || e->periodic[2]))
-            throw kp::error(KP_ERR_UNSUPPORTED, "kp_forward: peel-off images need a "
-                                                "non-periodic domain");
-        if (run->n_incl > 0 && out->image && (run->image_n < 1 || !(run->image_half > 0.0)))
-            throw kp::error(KP_ERR_ARG, "kp_forward: bad image geometry");
-        if (run->scatter_model != KP_THOMSON && run->scatter_model != KP_HG_RAYLEIGH)
-            throw kp::error(KP_ERR_ARG, "kp_forward: unknown scatter_model");
-        if (run->precision != KP_F64 && run->precision != KP_F32)
-            throw kp::error(KP_ERR_ARG, "kp_forward: unknown precision");
-        if (!(run->rr_survive > 0.0 && run->rr_survive <= 1.0))
-            throw kp::error(KP_ERR_ARG, "kp_forward: rr_survive must be in (0,1]");
+        check_forward_args(e, src, run, out, "kp_forward");
         device_guard dg(e->device);
         hipStream_t stream = (hipStream_t)stream_;
-
         kp::kparams P;
         uint64_t total = 0;
         fill_params(e, src, run, stream, P, total);
@@ -564,62 +783,8 @@ kp_status kp_forward(kp_engine *e, const kp_source *src, const kp_run *run, kp_t
         P.end_state = out->end_state;
         if (total == 0)
             return;
-        const kp_schedule &S = e->sched;
-        uint64_t blocks;
-        int T;
-        if (S.kind == KP_SCHED_HISTORY) {
-            T = 256;
-            blocks = (total + T - 1) / T;
-            if (blocks > 0x7fffffffull)
-                blocks = 0x7fffffffull;
-        } else {
-            T = S.block_threads;
-            blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
-            const uint64_t need = (total + T - 1) / T;
-            if (blocks > need)
-                blocks = need;
-        }
-        // replicas of the FP64 accumulators
-        const uint64_t nJ = (uint64_t)e->n_blocks * e->n3 * run->n_bands;
-        const uint64_t nI = (uint64_t)run->n_bands * run->n_incl * run->image_n * run->image_n * 3;
-        P.j_nrep = P.img_nrep = 1;
-        P.j_stride = nJ;
-        P.img_stride = nI;
-        if (P.J) {
-            P.j_nrep = pick_replicas(S.j_replicas, nJ, blocks, 160ull << 20);
-            if (P.j_nrep > 1) {
-                ensure_replicas(e->j_rep, e->j_rep_cap, (uint64_t)P.j_nrep * nJ, stream);
-                P.J = e->j_rep;
-            }
-        }
-        if (P.image && nI > 0) {
-            P.img_nrep = pick_replicas(S.img_replicas, nI, blocks, 32ull << 20);
-            if (P.img_nrep > 1) {
-                ensure_replicas(e->img_rep, e->img_rep_cap, (uint64_t)P.img_nrep * nI, stream);
-                P.image = e->img_rep;
-            }
-        }
-        if (S.kind == KP_SCHED_HISTORY) {
-            if (run->precision == KP_F64)
-                hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)blocks), dim3(T), 0, stream, P);
-            else
-                hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)blocks), dim3(T), 0, stream, P);
-        } else {
-            P.work = e->work;
-            P.chunk = (uint32_t)S.chunk;
-            P.batch = (uint32_t)S.event_batch;
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
-            hipLaunchKernelGGL(persist_variant<false>(run->precision, S.waves_per_simd),
-                               dim3((unsigned)blocks), dim3(T), 0, stream, P);
-        }
-        KP_HIP(hipGetLastError());
-        if (P.J && P.j_nrep > 1)
-            hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, stream, out->J, e->j_rep, nJ,
-                               P.j_nrep);
-        if (P.image && P.img_nrep > 1)
-            hipLaunchKernelGGL(k_fold, dim3(512), dim3(256), 0, stream, out->image, e->img_rep,
-                               nI, P.img_nrep);
-        KP_HIP(hipGetLastError());
+        const fwd_plan pl = plan_forward(e, e->fb, P, total, run, stream);
+        issue_forward(e->sched, e->fb, P, pl, run, stream, 1, 0);
     });
 }
 
@@ -691,7 +856,7 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         P.adj_lanes = lanes;
         P.jlin = grads->jlin;
         P.g_stride = nc;
-        P.g_nrep = pick_replicas(S.j_replicas, 3 * nc, blocks, 160ull << 20);
+        P.g_nrep = pick_replicas(S.j_replicas, 3 * nc, 3 * nc, blocks, 400ull << 20);
         double *gl = grads->d_logrho, *ga = grads->d_albedo, *gg = grads->d_g;
         if (P.g_nrep > 1) {
             ensure_replicas(e->g_rep, e->g_rep_cap, 3ull * P.g_nrep * nc, stream);
@@ -703,10 +868,11 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
             P.g_alb = ga;
             P.g_g = gg;
         }
-        P.work = e->work;
+        P.lds_n = 0;
+        P.work = e->fb.work;
         P.chunk = (uint32_t)S.chunk;
         P.batch = (uint32_t)S.event_batch;
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->work);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
         hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
                            dim3((unsigned)blocks), dim3(T), 0, stream, P);
         KP_HIP(hipGetLastError());
@@ -722,6 +888,205 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                                    P.g_nrep);
             KP_HIP(hipGetLastError());
         }
+    });
+}
+
+
+kp_status kp_graph_create(kp_engine *e, const kp_source *src, const kp_run *run, kp_tallies *out,
+                          int32_t n_chunks, kp_graph **g_out)
+{
+    return kp::guard([&] {
+        check_forward_args(e, src, run, out, "kp_graph_create");
+        if (!g_out || n_chunks < 1)
+            throw kp::error(KP_ERR_ARG, "kp_graph_create: need graph and n_chunks >= 1");
+        if (out->end_state && n_chunks > 1)
+            throw kp::error(KP_ERR_ARG, "kp_graph_create: end_state needs n_chunks == 1 (chunks "
+                                        "would overwrite each other's end states)");
+        device_guard dg(e->device);
+        hipStream_t cs = nullptr;
+        KP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        auto *g = new kp_graph();
+        g->e = e;
+        try {
+            kp::kparams P;
+            uint64_t total = 0;
+            fill_params(e, src, run, cs, P, total);
+            if (total == 0)
+                throw kp::error(KP_ERR_ARG, "kp_graph_create: no packets");
+            // private copy of the band table: later runs may upload another
+            const int nb = run->n_bands;
+            g->bands = dmalloc<kp::dband>(nb);
+            g->band_start = dmalloc<uint64_t>(nb + 1);
+            KP_HIP(hipMemcpyAsync(g->bands, e->bands_d, nb * sizeof(kp::dband),
+                                  hipMemcpyDeviceToDevice, cs));
+            KP_HIP(hipMemcpyAsync(g->band_start, e->band_start_d, (nb + 1) * sizeof(uint64_t),
+                                  hipMemcpyDeviceToDevice, cs));
+            P.bands = g->bands;
+            P.band_start = g->band_start;
+            P.J = out->J;
+            P.image = (run->n_incl > 0) ? out->image : nullptr;
+            P.counters = (unsigned long long *)out->counters;
+            P.end_state = out->end_state;
+            const fwd_plan pl = plan_forward(e, g->fb, P, total, run, cs);
+            KP_HIP(hipStreamSynchronize(cs));
+            uint64_t stride = 0;
+            for (int b = 0; b < nb; ++b)
+                stride = std::max<uint64_t>(stride, run->bands[b].n_packets);
+            KP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            try {
+                issue_forward(e->sched, g->fb, P, pl, run, cs, n_chunks, stride);
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(cs, &dead);
+                if (dead)
+                    (void)hipGraphDestroy(dead);
+                throw;
+            }
+            KP_HIP(hipStreamEndCapture(cs, &g->graph));
+            KP_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        } catch (...) {
+            kp_graph_destroy(g);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        (void)hipStreamDestroy(cs);
+        *g_out = g;
+    });
+}
+
+kp_status kp_graph_launch(kp_graph *g, void *stream)
+{
+    return kp::guard([&] {
+        if (!g || !g->exec)
+            throw kp::error(KP_ERR_ARG, "kp_graph_launch: null graph");
+        device_guard dg(g->e->device);
+        KP_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    });
+}
+
+void kp_graph_destroy(kp_graph *g)
+{
+    if (!g)
+        return;
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    dfree(g->bands);
+    dfree(g->band_start);
+    free_bufs(g->fb);
+    delete g;
+}
+
+kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
+                      uint64_t sample_packets, int32_t force, kp_schedule *best, void *stream_)
+{
+    return kp::guard([&] {
+        kp_tallies probe_shape{};
+        probe_shape.image = nullptr;
+        check_forward_args(e, src, run, &probe_shape, "kp_autotune");
+        if (!best || sample_packets < 1)
+            throw kp::error(KP_ERR_ARG, "kp_autotune: need best and sample_packets >= 1");
+        device_guard dg(e->device);
+        hipStream_t stream = (hipStream_t)stream_;
+        uint64_t total = 0;
+        for (int b = 0; b < run->n_bands; ++b)
+            total += run->bands[b].n_packets;
+        // signature of the workload shape the schedule is tuned for
+        std::ostringstream key;
+        key << run->precision << '/' << run->n_bands << '/' << run->n_incl << '/'
+            << run->image_n << '/' << run->scatter_model << '/' << e->n_blocks << '/'
+            << (int)std::log2((double)std::max<uint64_t>(total, 1));
+        auto hit = e->tuned.find(key.str());
+        if (!force && hit != e->tuned.end()) {
+            e->sched = hit->second;
+            *best = hit->second;
+            return;
+        }
+        // sample run: every band scaled to about sample_packets in total
+        std::vector<kp_band> bands(run->bands, run->bands + run->n_bands);
+        if (total == 0)
+            throw kp::error(KP_ERR_ARG, "kp_autotune: no packets");
+        for (auto &b : bands)
+            if (b.n_packets)
+                b.n_packets = std::max<uint64_t>(
+                    1, (uint64_t)((double)b.n_packets * (double)sample_packets / (double)total));
+        kp_run r = *run;
+        r.bands = bands.data();
+        const uint64_t nJ = (uint64_t)e->n_blocks * e->n3 * run->n_bands;
+        const uint64_t nI = (uint64_t)run->n_bands * run->n_incl * run->image_n * run->image_n * 3;
+        kp_tallies t{};
+        t.J = dmalloc<double>(nJ);
+        t.image = nI ? dmalloc<double>(nI) : nullptr;
+        t.counters = dmalloc<uint64_t>(KP_NCOUNTERS);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            dfree(t.J);
+            dfree(t.image);
+            dfree(t.counters);
+            if (e0)
+                (void)hipEventDestroy(e0);
+            if (e1)
+                (void)hipEventDestroy(e1);
+        };
+        const kp_schedule before = e->sched;
+        try {
+            KP_HIP(hipEventCreate(&e0));
+            KP_HIP(hipEventCreate(&e1));
+            auto time_it = [&](const kp_schedule &sc) -> float {
+                e->sched = sc;
+                kp::kparams P;
+                uint64_t tot = 0;
+                fill_params(e, src, &r, stream, P, tot);
+                P.J = t.J;
+                P.image = (r.n_incl > 0) ? t.image : nullptr;
+                P.counters = (unsigned long long *)t.counters;
+                P.end_state = nullptr;
+                const fwd_plan pl = plan_forward(e, e->fb, P, tot, &r, stream);
+                issue_forward(e->sched, e->fb, P, pl, &r, stream, 1, 0); // warm-up
+                KP_HIP(hipEventRecord(e0, stream));
+                issue_forward(e->sched, e->fb, P, pl, &r, stream, 2, 0);
+                KP_HIP(hipEventRecord(e1, stream));
+                KP_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                KP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                return ms;
+            };
+            kp_schedule cur = e->sched;
+            cur.kind = KP_SCHED_PERSISTENT;
+            float best_ms = time_it(cur);
+            // coordinate descent, one knob at a time
+            auto try_knob = [&](int kp_schedule::*f, std::initializer_list<int> vals) {
+                for (int v : vals) {
+                    if (cur.*f == v)
+                        continue;
+                    kp_schedule c = cur;
+                    c.*f = v;
+                    const float ms = time_it(c);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        cur = c;
+                    }
+                }
+            };
+            try_knob(&kp_schedule::j_replicas, {0, 1, 2, 4, 16});
+            try_knob(&kp_schedule::img_replicas, {0, 16, 64});
+            try_knob(&kp_schedule::event_batch, {4, 8, 16});
+            if (run->precision == KP_F64)
+                try_knob(&kp_schedule::waves_per_simd, {0, 3, 4});
+            else
+                try_knob(&kp_schedule::waves_per_simd, {0, 5});
+            try_knob(&kp_schedule::blocks_per_cu, {2, 4, 8});
+            try_knob(&kp_schedule::block_threads, {128, 256});
+            e->sched = cur;
+            e->tuned[key.str()] = cur;
+            *best = cur;
+        } catch (...) {
+            e->sched = before;
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 

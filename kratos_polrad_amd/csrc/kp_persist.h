@@ -175,10 +175,10 @@ template <class Pol, bool ADJ> __device__ __forceinline__ void emit_packet(const
 
 template <class Pol, bool ADJ>
 __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_counters &C,
-                                       int status)
+                                       int status, double *sh)
 {
     if constexpr (!ADJ && sizeof(typename Pol::real) == 4)
-        L.acc.flush(P, L.band);
+        L.acc.flush(P, L.band, sh);
     C.cross += L.ncross;
     C.peel += L.npeel;
     C.events += (uint32_t)L.events;
@@ -387,7 +387,7 @@ template <class Pol> __device__ __forceinline__ double peel_A(const kparams &P, 
 // then the next observer, or resume the main path (EMIT: march on;
 // SCATTER: pending scatter).
 template <class Pol, bool ADJ>
-__device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
+__device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L, double *sh)
 {
     using real = typename Pol::real;
     if constexpr (ADJ) {
@@ -422,9 +422,14 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
         }
         L.kslot += 1;
     } else if (P.image) {
+        real v[3];
+        long px;
         if constexpr (sizeof(real) == 8) {
             const double att = kp_exp(-L.tau) * KP_INV_4PI;
-            f64::deposit(P, L.band, L.o, L.sx, L.Pi * att, L.Pq * att, L.Pu * att);
+            px = f64::pixel(P, L.band, L.o, L.sx);
+            v[0] = L.Pi * att;
+            v[1] = L.Pq * att;
+            v[2] = L.Pu * att;
         } else {
             f32::ray_t tmp;
 #pragma unroll
@@ -434,7 +439,26 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L)
             float gx[3];
             f32::global_pos(P, tmp, gx);
             const float att = f32::fast_exp(-(float)L.tau) * f32::kInv4Pi;
-            f32::deposit(P, L.band, L.o, gx, L.Pi * att, L.Pq * att, L.Pu * att);
+            px = f32::pixel(P, L.band, L.o, gx);
+            v[0] = L.Pi * att;
+            v[1] = L.Pq * att;
+            v[2] = L.Pu * att;
+        }
+        if (px >= 0) {
+            // emission peel-offs of one band all land on the source pixel
+            // with equal values: summed in LDS, added once per workgroup
+            double *q = (sh && L.ev_kind == EV_EMIT)
+                            ? sh + P.n_bands + 3 * (L.band * P.n_incl + L.o)
+                            : img_base(P) + 3 * px;
+            if (sh && L.ev_kind == EV_EMIT) {
+                atomicAdd(q + 0, (double)v[0]);
+                atomicAdd(q + 1, (double)v[1]);
+                atomicAdd(q + 2, (double)v[2]);
+            } else {
+                atomic_add_f64(q + 0, (double)v[0]);
+                atomic_add_f64(q + 1, (double)v[1]);
+                atomic_add_f64(q + 2, (double)v[2]);
+            }
         }
     }
     L.o += 1;
@@ -548,9 +572,47 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
     }
 }
 
+// Adds the workgroup's LDS accumulators (see persist_body) to J and the
+// image.  Called by every thread of the workgroup after a barrier.
+template <class Pol> __device__ void flush_lds_acc(const kparams &P, const double *sh)
+{
+    using real = typename Pol::real;
+    for (int b = threadIdx.x; b < P.n_bands; b += blockDim.x)
+        if (P.J && sh[b] != 0.0)
+            atomic_add_f64(j_at(P, (uint64_t)P.src_gc, b), sh[b]);
+    const int n = P.n_bands * P.n_incl;
+    if (!P.image)
+        return;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double *v = sh + P.n_bands + 3 * k;
+        if (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0)
+            continue;
+        const int b = k / P.n_incl, o = k - b * P.n_incl;
+        // the emission point exactly as the packets saw it
+        typename Pol::ray r;
+        Pol::at_source(P, r);
+        real g[3];
+        Pol::gpos(P, r, g);
+        long px;
+        if constexpr (sizeof(real) == 8)
+            px = f64::pixel(P, b, o, g);
+        else
+            px = f32::pixel(P, b, o, g);
+        if (px < 0)
+            continue;
+        double *q = img_base(P) + 3 * px;
+        atomic_add_f64(q + 0, v[0]);
+        atomic_add_f64(q + 1, v[1]);
+        atomic_add_f64(q + 2, v[2]);
+    }
+}
+
+// sh: the workgroup's LDS accumulators (null when the launch has none):
+// [n_bands] J of the source cell, then [n_bands][n_incl][3] I,Q,U of the
+// emission peel-offs (all at the source pixel).
 template <class Pol, bool ADJ>
 __device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
-                             double &jsum)
+                             double &jsum, double *sh)
 {
     using real = typename Pol::real;
     lane_t<Pol> L;
@@ -578,13 +640,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
             // ---------------- action phase: every pending lane together
             if (L.mode == M_PEND && L.pa == PA_PEEL_END)
-                peel_end_stage<Pol, ADJ>(P, L);
+                peel_end_stage<Pol, ADJ>(P, L, sh);
             if (L.mode == M_PEND && L.pa == PA_INTERACT)
                 interact_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_SCATTER)
                 scatter_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
-                finish<Pol, ADJ>(P, L, C, L.status);
+                finish<Pol, ADJ>(P, L, C, L.status, sh);
             // refill idle lanes from the wave's slot range
             const uint64_t need = __ballot(L.mode == M_NEED);
             if (need) {
@@ -654,7 +716,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                             atomic_add_f64(g_base(P, P.g_logrho) + gc, -((kap * li) * L.B));
                     } else {
                         if (P.J)
-                            atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (li / V));
+                            j_deposit(P, sh, gc, L.band, L.I * (li / V));
                     }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
@@ -668,7 +730,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                         atomic_add_f64(g_base(P, P.g_logrho) + gc, -(dtau * L.B));
                 } else {
                     if (P.J)
-                        atomic_add_f64(j_base(P) + gc * P.n_bands + L.band, L.I * (l / V));
+                        j_deposit(P, sh, gc, L.band, L.I * (l / V));
                 }
             } else {
                 const float iV = f32::fast_rcp(P.vol32[L.cur.ib]);
@@ -679,7 +741,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                             atomic_add_f64(g_base(P, P.g_logrho) + gc,
                                            -((double)(kap * li) * L.B));
                     } else {
-                        L.acc.add(P, L.band, gc, L.I * li * iV);
+                        L.acc.add(P, L.band, gc, L.I * li * iV, sh);
                     }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
@@ -692,7 +754,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     if (L.pass == 2 && P.g_logrho && L.B != 0.0)
                         atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.B));
                 } else {
-                    L.acc.add(P, L.band, gc, L.I * l * iV);
+                    L.acc.add(P, L.band, gc, L.I * l * iV, sh);
                 }
             }
         } else {

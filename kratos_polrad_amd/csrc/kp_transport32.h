@@ -378,16 +378,18 @@ __device__ __forceinline__ void emit(const uint32_t rn[4], float *d, float *e1, 
 }
 
 // Kahan-compensated register accumulator for one cell of the J tally.
+// The source cell's sums go to the LDS accumulator `sh` when given.
 struct tally_acc {
     int64_t gc = -1;
     float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void flush(const kparams &P, int band)
+    __device__ __forceinline__ void flush(const kparams &P, int band, double *sh = nullptr)
     {
         if (gc >= 0 && P.J)
-            atomic_add_f64(j_base(P) + (size_t)gc * P.n_bands + band, (double)s - (double)c);
+            j_deposit(P, sh, gc, band, (double)s - (double)c);
         gc = -1;
     }
-    __device__ __forceinline__ void add(const kparams &P, int band, int64_t cell, float v)
+    __device__ __forceinline__ void add(const kparams &P, int band, int64_t cell, float v,
+                                        double *sh = nullptr)
     {
         if (cell == gc) {
             const float y = v - c;
@@ -395,7 +397,7 @@ struct tally_acc {
             c = (t - s) - y;
             s = t;
         } else {
-            flush(P, band);
+            flush(P, band, sh);
             gc = cell;
             s = v;
             c = 0.0f;

@@ -427,7 +427,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
                 if (li > l)
                     li = l;
                 if (P.J)
-                    atomic_add_f64(j_base(P) + gc * P.n_bands + band, I * (li / V));
+                    atomic_add_f64(j_at(P, gc, band), I * (li / V));
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
                     r.x[a] = r.x[a] + li * r.d[a];
@@ -436,7 +436,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
             }
             tau_rem = tau_rem - dtau;
             if (P.J)
-                atomic_add_f64(j_base(P) + gc * P.n_bands + band, I * (l / V));
+                atomic_add_f64(j_at(P, gc, band), I * (l / V));
             bool reload;
             const int st = cross_face(P, r, ap, l, reload);
             if (st == ST_ESCAPED) {

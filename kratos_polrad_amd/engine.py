@@ -116,6 +116,25 @@ class Engine:
         _abi.check(_abi.lib().kp_get_schedule(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _abi.Schedule._fields_}
 
+    def autotune(self, source_x, run: "RunSpec", sample_packets=2_000_000, force=False,
+                 weight=1.0, stream=None) -> dict:
+        """Time candidate schedules on a packet sample and install the fastest."""
+        t = self._torch
+        r, keep = run.to_c()
+        best = _abi.Schedule()
+        src = source_c(source_x, weight)
+        if stream is None:
+            stream = t.cuda.current_stream(self.device)
+        _abi.check(_abi.lib().kp_autotune(self._h, C.byref(src), C.byref(r), int(sample_packets),
+                                           int(bool(force)), C.byref(best),
+                                           C.c_void_p(stream.cuda_stream)))
+        del keep
+        return {k: getattr(best, k) for k, _ in _abi.Schedule._fields_}
+
+    def graph(self, source_x, run: "RunSpec", outputs: dict, n_chunks=1, weight=1.0):
+        """Capture a forward run (n_chunks chunks) into a replayable hipGraph."""
+        return Graph(self, source_x, run, outputs, n_chunks, weight)
+
     def set_media(self, kappa_rho, albedo, g=None) -> None:
         n = self.mesh.n_cells
         k = np.ascontiguousarray(np.asarray(kappa_rho, dtype=np.float32).reshape(n))
@@ -187,3 +206,33 @@ class Engine:
                                           C.c_void_p(stream.cuda_stream)))
         del keep
         return grads
+
+
+class Graph:
+    """A captured forward run; launch() replays it (tallies accumulate)."""
+
+    def __init__(self, engine, source_x, run, outputs, n_chunks=1, weight=1.0):
+        self.engine = engine
+        self.outputs = outputs
+        r, keep = run.to_c()
+        tal = _abi.Tallies()
+        for key in ("J", "image", "counters", "end_state"):
+            v = outputs.get(key)
+            setattr(tal, key, v.data_ptr() if v is not None else None)
+        src = source_c(source_x, weight)
+        self._h = C.c_void_p()
+        _abi.check(_abi.lib().kp_graph_create(engine.handle, C.byref(src), C.byref(r),
+                                               C.byref(tal), int(n_chunks), C.byref(self._h)))
+        del keep
+
+    def launch(self, stream=None):
+        t = self.engine._torch
+        if stream is None:
+            stream = t.cuda.current_stream(self.engine.device)
+        _abi.check(_abi.lib().kp_graph_launch(self._h, C.c_void_p(stream.cuda_stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _abi.lib().kp_graph_destroy(h)
+            self._h = None
