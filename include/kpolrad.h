@@ -216,7 +216,8 @@ typedef struct kp_schedule {
      * ends / packet ends together once this many lanes wait (1..64) */
     int32_t event_batch;
     /* register budget of the persistent kernel: minimum waves per SIMD the
-     * compiler must fit (0 = its own choice; FP32: 0,5,6; FP64: 0,3,4) */
+     * compiler must fit (0 = default: FP32 4, FP64 unconstrained; 1 =
+     * unconstrained; FP32: 4,5,6; FP64: 3,4) */
     int32_t waves_per_simd;
 } kp_schedule;
 

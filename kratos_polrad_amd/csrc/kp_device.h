@@ -103,10 +103,12 @@ __device__ __forceinline__ double *j_base(const kparams &P)
 {
     return P.J + (uint64_t)(blockIdx.x % P.j_nrep) * P.j_stride;
 }
-// address of the J tally of global cell gc in band b (this workgroup's replica)
+// address of the J tally of global cell gc in band b (this workgroup's replica);
+// cell ids and strides are below 2^32, so the products are 32x32->64 bit
 __device__ __forceinline__ double *j_at(const kparams &P, uint64_t gc, int b)
 {
-    return j_base(P) + gc * P.j_cs + (uint64_t)b * P.j_bs;
+    return j_base(P) + (uint64_t)(uint32_t)gc * (uint32_t)P.j_cs +
+           (uint64_t)(uint32_t)b * (uint32_t)P.j_bs;
 }
 __device__ __forceinline__ double *img_base(const kparams &P)
 {
@@ -134,15 +136,19 @@ __device__ __forceinline__ int band_of(const kparams &P, uint64_t slot)
     return lo;
 }
 
-// J deposit of v into cell gc of band b.  `sh` (persistent launches) is the
-// workgroup's LDS accumulator block: its first n_bands doubles collect the
-// source cell's J -- every packet starts in that cell, so those deposits
-// would otherwise serialise on one global address.
-__device__ __forceinline__ void j_deposit(const kparams &P, double *sh, int64_t gc, int b,
+// Dynamic LDS of persistent forward launches: the workgroup's accumulators,
+// [n_bands] J of the source cell, then [n_bands][n_incl][3] I,Q,U of the
+// emission peel-offs.  Every packet starts in the source cell and its
+// emission peel-off lands on the source pixel, so those deposits would
+// otherwise serialise on a few global addresses.
+extern __shared__ double kp_lds[];
+
+// J deposit of v into cell gc of band b (the source cell to LDS when `lds`).
+__device__ __forceinline__ void j_deposit(const kparams &P, bool lds, int64_t gc, int b,
                                           double v)
 {
-    if (sh && gc == P.src_gc)
-        atomicAdd(sh + b, v);
+    if (lds && gc == P.src_gc)
+        atomicAdd(&kp_lds[b], v);
     else
         atomic_add_f64(j_at(P, (uint64_t)gc, b), v);
 }

@@ -243,24 +243,23 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
 template <class Pol, int WPS, bool ADJ>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
-    extern __shared__ double k_sh[];
-    double *sh = nullptr;
+    bool lds = false;
     if constexpr (!ADJ) {
         if (P.lds_n > 0) {
             for (int k = threadIdx.x; k < P.lds_n; k += blockDim.x)
-                k_sh[k] = 0.0;
+                kp::kp_lds[k] = 0.0;
             __syncthreads();
-            sh = k_sh;
+            lds = true;
         }
     }
     kp::lane_counters C;
     double jsum = 0.0;
-    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum, sh);
+    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum, lds);
     flush_counters(P, C);
     if constexpr (!ADJ) {
         if (P.lds_n > 0) {
             __syncthreads();
-            kp::flush_lds_acc<Pol>(P, k_sh);
+            kp::flush_lds_acc<Pol>(P);
         }
     }
     if constexpr (ADJ) {
@@ -283,11 +282,13 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
             return k_persist<kp::pol64, 4, ADJ>;
         return k_persist<kp::pol64, 1, ADJ>;
     }
+    if (wps == 1)
+        return k_persist<kp::pol32, 1, ADJ>;
     if (wps == 5)
         return k_persist<kp::pol32, 5, ADJ>;
     if (wps == 6)
         return k_persist<kp::pol32, 6, ADJ>;
-    return k_persist<kp::pol32, 1, ADJ>;
+    return k_persist<kp::pol32, 4, ADJ>; // 128 VGPRs, no spills
 }
 
 // One thread per packet, both replay passes in that thread (FP64).
@@ -669,6 +670,8 @@ kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **ou
             throw kp::error(KP_ERR_ARG, "kp_engine_create: null argument");
         if (m->n_blocks < 1 || m->n_cell < 2 || (m->n_cell & 1))
             throw kp::error(KP_ERR_ARG, "kp_engine_create: bad block table");
+        if ((uint64_t)m->n_blocks * m->n_cell * m->n_cell * m->n_cell >= (1ull << 31))
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_engine_create: more than 2^31 cells");
         int ndev = 0;
         KP_HIP(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev)
@@ -1075,7 +1078,7 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
             if (run->precision == KP_F64)
                 try_knob(&kp_schedule::waves_per_simd, {0, 3, 4});
             else
-                try_knob(&kp_schedule::waves_per_simd, {0, 5});
+                try_knob(&kp_schedule::waves_per_simd, {0, 1, 5});
             try_knob(&kp_schedule::blocks_per_cu, {2, 4, 8});
             try_knob(&kp_schedule::block_threads, {128, 256});
             e->sched = cur;
@@ -1103,9 +1106,9 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: blocks_per_cu must be in [1,64]");
         if (s->chunk < 1 || s->chunk > 4096)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: chunk must be in [1,4096]");
-        if (s->waves_per_simd != 0 && s->waves_per_simd != 3 && s->waves_per_simd != 4 &&
-            s->waves_per_simd != 5 && s->waves_per_simd != 6)
-            throw kp::error(KP_ERR_ARG, "kp_set_schedule: waves_per_simd must be 0, 3, 4, 5 or 6");
+        if (s->waves_per_simd < 0 || s->waves_per_simd > 6 || s->waves_per_simd == 2)
+            throw kp::error(KP_ERR_ARG,
+                            "kp_set_schedule: waves_per_simd must be 0, 1, 3, 4, 5 or 6");
         if (s->event_batch < 1 || s->event_batch > 64)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: event_batch must be in [1,64]");
         if (s->j_replicas < 0 || s->j_replicas > 256 || s->img_replicas < 0 ||

@@ -149,6 +149,7 @@ template <class Pol> struct lane_t {
     float kcache;
     int events, steps;
     uint32_t ncross, npeel;
+    real ks;            // extinction scale of the packet's band
     f32::tally_acc acc; // FP32: Kahan register accumulator of the current cell
     // path replay (adjoint launches only)
     int pass;         // 1: collect peel-off contributions A_k; 2: gradients
@@ -175,10 +176,10 @@ template <class Pol, bool ADJ> __device__ __forceinline__ void emit_packet(const
 
 template <class Pol, bool ADJ>
 __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_counters &C,
-                                       int status, double *sh)
+                                       int status, bool lds)
 {
     if constexpr (!ADJ && sizeof(typename Pol::real) == 4)
-        L.acc.flush(P, L.band, sh);
+        L.acc.flush(P, L.band, lds);
     C.cross += L.ncross;
     C.peel += L.npeel;
     C.events += (uint32_t)L.events;
@@ -387,7 +388,7 @@ template <class Pol> __device__ __forceinline__ double peel_A(const kparams &P, 
 // then the next observer, or resume the main path (EMIT: march on;
 // SCATTER: pending scatter).
 template <class Pol, bool ADJ>
-__device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L, double *sh)
+__device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L, bool lds)
 {
     using real = typename Pol::real;
     if constexpr (ADJ) {
@@ -447,14 +448,13 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
         if (px >= 0) {
             // emission peel-offs of one band all land on the source pixel
             // with equal values: summed in LDS, added once per workgroup
-            double *q = (sh && L.ev_kind == EV_EMIT)
-                            ? sh + P.n_bands + 3 * (L.band * P.n_incl + L.o)
-                            : img_base(P) + 3 * px;
-            if (sh && L.ev_kind == EV_EMIT) {
-                atomicAdd(q + 0, (double)v[0]);
-                atomicAdd(q + 1, (double)v[1]);
-                atomicAdd(q + 2, (double)v[2]);
+            if (lds && L.ev_kind == EV_EMIT) {
+                const int k = P.n_bands + 3 * (L.band * P.n_incl + L.o);
+                atomicAdd(&kp_lds[k + 0], (double)v[0]);
+                atomicAdd(&kp_lds[k + 1], (double)v[1]);
+                atomicAdd(&kp_lds[k + 2], (double)v[2]);
             } else {
+                double *q = img_base(P) + 3 * px;
                 atomic_add_f64(q + 0, (double)v[0]);
                 atomic_add_f64(q + 1, (double)v[1]);
                 atomic_add_f64(q + 2, (double)v[2]);
@@ -489,6 +489,7 @@ template <class Pol, bool ADJ>
 __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
 {
     L.band = band_of(P, L.slot);
+    L.ks = (typename Pol::real)P.bands[L.band].kappa_scale;
     L.pid = P.packet_offset + (L.slot - P.band_start[L.band]);
     uint32_t rn[4] = {(uint32_t)L.pid, (uint32_t)(L.pid >> 32), 0u, 0u};
     kp_philox4x32_10(rn, P.seed, (uint32_t)L.band);
@@ -572,10 +573,11 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
     }
 }
 
-// Adds the workgroup's LDS accumulators (see persist_body) to J and the
-// image.  Called by every thread of the workgroup after a barrier.
-template <class Pol> __device__ void flush_lds_acc(const kparams &P, const double *sh)
+// Adds the workgroup's LDS accumulators (kp_lds) to J and the image.
+// Called by every thread of the workgroup after a barrier.
+template <class Pol> __device__ void flush_lds_acc(const kparams &P)
 {
+    const double *sh = kp_lds;
     using real = typename Pol::real;
     for (int b = threadIdx.x; b < P.n_bands; b += blockDim.x)
         if (P.J && sh[b] != 0.0)
@@ -607,12 +609,10 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P, const doubl
     }
 }
 
-// sh: the workgroup's LDS accumulators (null when the launch has none):
-// [n_bands] J of the source cell, then [n_bands][n_incl][3] I,Q,U of the
-// emission peel-offs (all at the source pixel).
+// lds: the launch gives each workgroup the kp_lds accumulators.
 template <class Pol, bool ADJ>
 __device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
-                             double &jsum, double *sh)
+                             double &jsum, bool lds)
 {
     using real = typename Pol::real;
     lane_t<Pol> L;
@@ -640,13 +640,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
             // ---------------- action phase: every pending lane together
             if (L.mode == M_PEND && L.pa == PA_PEEL_END)
-                peel_end_stage<Pol, ADJ>(P, L, sh);
+                peel_end_stage<Pol, ADJ>(P, L, lds);
             if (L.mode == M_PEND && L.pa == PA_INTERACT)
                 interact_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_SCATTER)
                 scatter_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
-                finish<Pol, ADJ>(P, L, C, L.status, sh);
+                finish<Pol, ADJ>(P, L, C, L.status, lds);
             // refill idle lanes from the wave's slot range
             const uint64_t need = __ballot(L.mode == M_NEED);
             if (need) {
@@ -697,10 +697,15 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             L.status = KP_END_LOST;
             continue;
         }
-        const int64_t gc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
-        const real ks = (real)P.bands[L.band].kappa_scale;
-        const float kraw = (gc == L.kgc) ? L.kcache : P.kappa[gc];
-        const real kap = (real)kraw * ks;
+        // the cell and its kappa_rho were fetched by the previous crossing
+        // unless the ray was (re)started since
+        if (L.kgc < 0) {
+            L.kgc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
+            L.kcache = P.kappa[L.kgc];
+        }
+        const int64_t gc = L.kgc;
+        const float kraw = L.kcache;
+        const real kap = (real)kraw * L.ks;
         const real dtau = kap * l;
         if (is_main) {
             L.ncross += 1;
@@ -716,7 +721,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                             atomic_add_f64(g_base(P, P.g_logrho) + gc, -((kap * li) * L.B));
                     } else {
                         if (P.J)
-                            j_deposit(P, sh, gc, L.band, L.I * (li / V));
+                            j_deposit(P, lds, gc, L.band, L.I * (li / V));
                     }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
@@ -730,10 +735,10 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                         atomic_add_f64(g_base(P, P.g_logrho) + gc, -(dtau * L.B));
                 } else {
                     if (P.J)
-                        j_deposit(P, sh, gc, L.band, L.I * (l / V));
+                        j_deposit(P, lds, gc, L.band, L.I * (l / V));
                 }
             } else {
-                const float iV = f32::fast_rcp(P.vol32[L.cur.ib]);
+                const float iV = L.cur.iv;
                 if (kap > 0.0f && (double)dtau >= L.tau) {
                     const float li = fminf((float)L.tau * f32::fast_rcp(kap), l);
                     if constexpr (ADJ) {
@@ -741,7 +746,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                             atomic_add_f64(g_base(P, P.g_logrho) + gc,
                                            -((double)(kap * li) * L.B));
                     } else {
-                        L.acc.add(P, L.band, gc, L.I * li * iV, sh);
+                        L.acc.add(P, L.band, gc, L.I * li * iV, lds);
                     }
                     Pol::advance(L.cur, li);
                     L.gc = gc;
@@ -754,7 +759,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     if (L.pass == 2 && P.g_logrho && L.B != 0.0)
                         atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.B));
                 } else {
-                    L.acc.add(P, L.band, gc, L.I * l * iV, sh);
+                    L.acc.add(P, L.band, gc, L.I * l * iV, lds);
                 }
             }
         } else {

@@ -50,6 +50,7 @@ struct ray_t {
     float cpl[3]; // cell units per physical length along a (d/h)
     int i[3];
     int ib;
+    float iv;     // 1 / cell volume of block ib
 };
 
 __device__ __forceinline__ void set_scales(const kparams &P, ray_t &r)
@@ -60,6 +61,7 @@ __device__ __forceinline__ void set_scales(const kparams &P, ray_t &r)
         r.cpl[a] = r.d[a] * fast_rcp(h);
         r.lpc[a] = (r.d[a] != 0.0f) ? h * fast_rcp(r.d[a]) : kInf;
     }
+    r.iv = fast_rcp(P.vol32[r.ib]);
 }
 
 __device__ __forceinline__ int get_dl(const ray_t &r, float &lmin)
@@ -67,12 +69,10 @@ __device__ __forceinline__ int get_dl(const ray_t &r, float &lmin)
     float t[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (r.d[a] == 0.0f) {
-            t[a] = kInf;
-        } else {
-            const float face = (float)(r.d[a] > 0.0f ? r.i[a] + 1 : r.i[a]);
-            t[a] = fmaxf((face - r.x[a]) * r.lpc[a], 0.0f);
-        }
+        // computed for every axis and selected, so the wave does not branch
+        const float face = (float)(r.i[a] + (r.d[a] > 0.0f ? 1 : 0));
+        const float ta = fmaxf((face - r.x[a]) * r.lpc[a], 0.0f);
+        t[a] = (r.d[a] == 0.0f) ? kInf : ta;
     }
     int ap = -1;
     float best = kInf;
@@ -136,6 +136,8 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, fl
     r.ib = nb;
     const float half = (float)(n / 2);
     const float sc = dl < 0 ? 2.0f : (dl > 0 ? 0.5f : 1.0f);
+    // cell volumes of neighbouring levels differ by exactly 8
+    r.iv *= dl < 0 ? 0.125f : (dl > 0 ? 8.0f : 1.0f);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         r.lpc[q] *= sc;
@@ -378,18 +380,18 @@ __device__ __forceinline__ void emit(const uint32_t rn[4], float *d, float *e1, 
 }
 
 // Kahan-compensated register accumulator for one cell of the J tally.
-// The source cell's sums go to the LDS accumulator `sh` when given.
+// The source cell's sums go to the LDS accumulators when `lds`.
 struct tally_acc {
     int64_t gc = -1;
     float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void flush(const kparams &P, int band, double *sh = nullptr)
+    __device__ __forceinline__ void flush(const kparams &P, int band, bool lds = false)
     {
         if (gc >= 0 && P.J)
-            j_deposit(P, sh, gc, band, (double)s - (double)c);
+            j_deposit(P, lds, gc, band, (double)s - (double)c);
         gc = -1;
     }
     __device__ __forceinline__ void add(const kparams &P, int band, int64_t cell, float v,
-                                        double *sh = nullptr)
+                                        bool lds = false)
     {
         if (cell == gc) {
             const float y = v - c;
@@ -397,7 +399,7 @@ struct tally_acc {
             c = (t - s) - y;
             s = t;
         } else {
-            flush(P, band, sh);
+            flush(P, band, lds);
             gc = cell;
             s = v;
             c = 0.0f;
@@ -481,7 +483,7 @@ __device__ void packet(const kparams &P, int band, uint64_t pid, uint64_t slot, 
             const int64_t gc = (int64_t)r.ib * P.n3 + cell;
             const float kap = P.kappa[gc] * ks;
             const float dtau = kap * l;
-            const float iV = fast_rcp(P.vol32[r.ib]);
+            const float iV = r.iv;
             ncross += 1;
             steps -= 1;
             if (kap > 0.0f && (double)dtau >= tau_rem) {
