@@ -256,6 +256,43 @@ kp_status kp_graph_create(kp_engine *eng, const kp_source *src, const kp_run *ru
 kp_status kp_graph_launch(kp_graph *graph, void *stream);
 void kp_graph_destroy(kp_graph *graph);
 
+/* ================================================================== */
+/* Packet-pool compaction and per-destination binning                  */
+/* ================================================================== */
+
+/* Device workspace (bytes) that kp_pool_keep / kp_bin_by_key need for n
+ * records.  No reference counterpart (the reference allocates inside its
+ * drivers, particle_pool.h:77-114, comm.h:114). */
+size_t kp_compact_workspace_bytes(int64_t n, int32_t n_bins);
+
+/* pool_t::keep_pool + ker_pool_relocate (types/particle_pool.h:191-248):
+ * records i with keep[i] == 0 are removed from the device pool of n records
+ * of unit_bytes (multiple of 4); the survivors end up in [0, n_keep) and
+ * only tail survivors move: the j-th hole of [0, n_keep) (ascending)
+ * receives the j-th survivor of [n_keep, n) counted from the end (the
+ * reference pairs them in atomic completion order).  *n_keep_dev (device
+ * int64) receives n_keep.  Asynchronous on `stream`. */
+kp_status kp_pool_keep(void *pool, int64_t n, int32_t unit_bytes, const uint8_t *keep,
+                       int64_t *n_keep_dev, void *workspace, size_t workspace_bytes,
+                       void *stream);
+
+/* comm_t::reg (boundary/comm.h:118-127): record i with keys[i] in
+ * [0, n_bins) gets slot offsets[keys[i]] + (its rank among the records of
+ * that key in index order) -- a stable counting sort instead of the
+ * reference's atomicAdd sequence numbers; other keys stay (slot -1).
+ * offsets: device int64 [n_bins+1] (offsets[n_bins] = records binned);
+ * order: device int32 [n] (order[s] = record in slot s, s < offsets[n_bins]),
+ * slot: device int32 [n]; order and slot may be NULL.  n < 2^31. */
+kp_status kp_bin_by_key(const int32_t *keys, int64_t n, int32_t n_bins, int64_t *offsets,
+                        int32_t *order, int32_t *slot, void *workspace,
+                        size_t workspace_bytes, void *stream);
+
+/* comm_t::cpy_d (boundary/comm.h:226-239): dst[s] = src[order[s]] for
+ * s < min(*n_dev, n_max), records of unit_bytes (multiple of 4); n_dev is a
+ * device int64 (e.g. offsets + n_bins of kp_bin_by_key). */
+kp_status kp_gather(const void *src, int32_t unit_bytes, const int32_t *order,
+                    const int64_t *n_dev, int64_t n_max, void *dst, void *stream);
+
 /* Device gradient outputs of kp_adjoint, per cell (each may be NULL);
  * kp_adjoint ADDS into them.  jlin (device scalar) receives the primal
  * value sum(dL_dimage * image) of the replayed histories. */

@@ -141,6 +141,10 @@ SIGNATURES = {
     "kp_graph_create": (C.c_int, [_P, C.POINTER(Source), C.POINTER(Run), C.POINTER(Tallies),
                                   C.c_int32, C.POINTER(_P)]),
     "kp_graph_launch": (C.c_int, [_P, _P]),
+    "kp_compact_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "kp_pool_keep": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, C.c_size_t, _P]),
+    "kp_bin_by_key": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_size_t, _P]),
+    "kp_gather": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, _P, _P]),
     "kp_graph_destroy": (None, [_P]),
     "kp_get_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),
 }

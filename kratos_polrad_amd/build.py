@@ -26,6 +26,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-mcode-object-version=5", "-Xclang", "-targ
 
 SOURCES = [
     ("kp_engine.hip", True),
+    ("kp_compact.hip", True),
     ("kp_mesh.cpp", False),
     ("kp_error.cpp", False),
 ]

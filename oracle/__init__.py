@@ -66,6 +66,12 @@ def lib():
         L.or_scatter_stokes.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D3, D3, D3]
         L.or_peel_stokes.restype = None
         L.or_peel_stokes.argtypes = [C.c_int, C.c_double, C.c_double, D3, D3, D3, D3, D3, D3]
+        L.or_pool_keep.restype = C.c_int64
+        L.or_pool_keep.argtypes = [P, C.c_int64, C.c_int64, P]
+        L.or_bin_by_key.restype = C.c_int64
+        L.or_bin_by_key.argtypes = [P, C.c_int64, C.c_int32, P, P, P]
+        L.or_gather.restype = None
+        L.or_gather.argtypes = [P, C.c_int64, P, C.c_int64, P]
         _lib = L
     return _lib
 
@@ -168,3 +174,34 @@ def adjoint(mesh, kappa, albedo, g, source_x, run_spec, dL_dimage, weight=1.0, n
     if rc != 0:
         raise RuntimeError(f"or_adjoint failed: {rc}")
     return dl, da, dg, jl.value
+
+
+def pool_keep(pool: np.ndarray, keep: np.ndarray):
+    """In-place keep_pool compaction of the records (rows) of ``pool``; returns
+    (compacted copy, n_keep)."""
+    p = np.ascontiguousarray(pool).copy()
+    k = np.ascontiguousarray(keep, dtype=np.uint8)
+    n = p.shape[0]
+    unit = p.strides[0] if n else 0
+    nk = lib().or_pool_keep(_ptr(p), n, unit, _ptr(k))
+    return p, int(nk)
+
+
+def bin_by_key(keys: np.ndarray, n_bins: int):
+    """Stable per-destination binning: (offsets[n_bins+1], order[n_sent], slot[n])."""
+    k = np.ascontiguousarray(keys, dtype=np.int32)
+    n = k.shape[0]
+    offsets = np.zeros(n_bins + 1, dtype=np.int64)
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    slot = np.zeros(max(n, 1), dtype=np.int32)
+    ns = lib().or_bin_by_key(_ptr(k), n, int(n_bins), _ptr(offsets), _ptr(order), _ptr(slot))
+    return offsets, order[:ns].copy(), slot[:n].copy()
+
+
+def gather(src: np.ndarray, order: np.ndarray):
+    s = np.ascontiguousarray(src)
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    out = np.empty((len(o),) + s.shape[1:], dtype=s.dtype)
+    unit = s.strides[0] if len(s) else 0
+    lib().or_gather(_ptr(s), unit, _ptr(o), len(o), _ptr(out))
+    return out

@@ -72,6 +72,12 @@ void or_peel_stokes(int model, double g, double pmax, const double d[3], const d
 
 /* Leaf arithmetic exposed for pinning against libm / published vectors. */
 void or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
+
+/* Pool compaction and per-destination binning (kp_compact_oracle.c). */
+int64_t or_pool_keep(void *pool, int64_t n, int64_t unit, const uint8_t *keep);
+int64_t or_bin_by_key(const int32_t *key, int64_t n, int32_t n_bins, int64_t *offsets,
+                      int32_t *order, int32_t *slot);
+void or_gather(const void *src, int64_t unit, const int32_t *order, int64_t n, void *dst);
 double or_log(double x);
 double or_exp(double x);
 void or_sincos2pi(double u, double *s, double *c);
