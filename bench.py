@@ -37,11 +37,12 @@ def parse():
     p.add_argument("--workload", default="config2", choices=["config1", "config2"])
     p.add_argument("--packets", type=float, default=0, help="packets per step")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--skip-f64", action="store_true")
     p.add_argument("--skip-adjoint", action="store_true")
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
+    p.add_argument("--skip-compaction", action="store_true")
     p.add_argument("--batched-scale", type=float, default=1.0,
                    help="scale of the per-band packet counts of the 64-band workload")
     p.add_argument("--copy-calibration", action="store_true",
@@ -190,6 +191,73 @@ def batched(steps, scale):
     return res
 
 
+def compaction(n=100_000_000, unit=64, n_bins=2234, p_keep=0.7):
+    """Pool compaction and per-destination binning + gather of n packet
+    records of `unit` bytes (device-resident inputs), timed with HIP events
+    on the launch stream.  GB/s are algorithmic bytes: keep flags read once
+    plus moved records read and written (keep); keys read and order written
+    (bin); records read and written plus order read (gather)."""
+    import ctypes as C
+
+    import torch
+
+    from kratos_polrad_amd import _abi
+
+    L = _abi.lib()
+    s = torch.cuda.current_stream()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    pool = torch.randint(0, 2**31 - 1, (n, unit // 4), dtype=torch.int32, device="cuda",
+                         generator=g)
+    keep = (torch.rand(n, device="cuda", generator=g) < p_keep).to(torch.uint8)
+    keys = torch.randint(-1, n_bins, (n,), dtype=torch.int32, device="cuda", generator=g)
+    wb = int(L.kp_compact_workspace_bytes(n, n_bins))
+    ws = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    nk = torch.zeros(1, dtype=torch.int64, device="cuda")
+    offs = torch.zeros(n_bins + 1, dtype=torch.int64, device="cuda")
+    order = torch.empty(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(pool)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    st = C.c_void_p(s.cuda_stream)
+    res = {"records": n, "unit_bytes": unit, "n_bins": n_bins, "p_keep": p_keep}
+
+    def timed(fn, reps=3):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    work = torch.empty_like(pool)
+
+    def keep_once():
+        work.copy_(pool)  # restore the pool; timed separately below
+        _abi.check(L.kp_pool_keep(P(work), n, unit, P(keep), P(nk), P(ws), wb, st))
+
+    t_copy = timed(lambda: work.copy_(pool))
+    t_keep = timed(keep_once) - t_copy
+    n_keep = int(nk.item())
+    n_move = int(keep[n_keep:].sum().item())
+    b_keep = n + 2.0 * unit * n_move
+    res["pool_keep"] = {"ms": 1e3 * t_keep, "moved": n_move, "GB_s": b_keep / t_keep / 1e9}
+    t_bin = timed(lambda: _abi.check(L.kp_bin_by_key(P(keys), n, n_bins, P(offs), P(order),
+                                                      None, P(ws), wb, st)))
+    res["bin_by_key"] = {"ms": 1e3 * t_bin, "binned": int(offs[-1].item()),
+                         "GB_s": 8.0 * n / t_bin / 1e9}
+    n_sent = int(offs[-1].item())
+    t_g = timed(lambda: _abi.check(L.kp_gather(P(pool), unit, P(order),
+                                                C.c_void_p(offs.data_ptr() + 8 * n_bins), n,
+                                                P(dst), st)))
+    res["gather"] = {"ms": 1e3 * t_g, "GB_s": (2.0 * unit + 4.0) * n_sent / t_g / 1e9}
+    del pool, work, dst, ws
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -296,6 +364,9 @@ def main():
 
     if not args.skip_adjoint:
         res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))
+
+    if not args.skip_compaction:
+        res["compaction"] = compaction()
 
     if not args.no_cpu:
         import oracle

@@ -116,7 +116,22 @@ class Grads(C.Structure):
     ]
 
 
-# Every entry point declared in include/kpolrad.h, with its ctypes signature.
+class ModuleInfo(C.Structure):
+    _fields_ = [
+        ("n_blocks", C.c_int64),
+        ("n_cells", C.c_int64),
+        ("n_cell", C.c_int32),
+        ("n_bands", C.c_int32),
+        ("n_incl", C.c_int32),
+        ("image_n", C.c_int32),
+        ("n_par", C.c_int64),
+        ("n_cycle", C.c_int32),
+        ("initialised", C.c_int32),
+        ("packets_done", C.c_int64),
+    ]
+
+
+# Every entry point declared in include/*.h, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
     "kp_last_error": (C.c_char_p, []),
@@ -142,6 +157,15 @@ SIGNATURES = {
                                   C.c_int32, C.POINTER(_P)]),
     "kp_graph_launch": (C.c_int, [_P, _P]),
     "kp_compact_workspace_bytes": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "kp_module_create": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "kp_module_destroy": (None, [_P]),
+    "kp_module_init": (C.c_int, [_P, C.c_int32]),
+    "kp_module_set_media": (C.c_int, [_P, _P, _P, _P]),
+    "kp_module_step": (C.c_int, [_P, _P]),
+    "kp_module_save": (C.c_int, [_P, C.c_char_p]),
+    "kp_module_get_info": (C.c_int, [_P, C.POINTER(ModuleInfo)]),
+    "kp_module_tallies": (C.c_int, [_P, _P, _P, _P]),
+    "kp_module_mesh": (C.c_int, [_P, C.POINTER(_P)]),
     "kp_pool_keep": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, C.c_size_t, _P]),
     "kp_bin_by_key": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_size_t, _P]),
     "kp_gather": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, _P, _P]),

@@ -28,6 +28,7 @@ SOURCES = [
     ("kp_engine.hip", True),
     ("kp_compact.hip", True),
     ("kp_mesh.cpp", False),
+    ("kp_module.cpp", False),
     ("kp_error.cpp", False),
 ]
 
