@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
     p.add_argument("--skip-compaction", action="store_true")
+    p.add_argument("--fd-cells", type=int, default=64,
+                   help="cells of the per-cell finite-difference check of the adjoint (0: off)")
     p.add_argument("--batched-scale", type=float, default=1.0,
                    help="scale of the per-band packet counts of the 64-band workload")
     p.add_argument("--copy-calibration", action="store_true",
@@ -189,6 +191,121 @@ def batched(steps, scale):
             "worst_band": dict(zip("IQU", e.max(axis=0).tolist()))}
     res["image_error_ratio_mixed_over_fp64_median"] = float(np.median(e32[:, 0] / e64[:, 0]))
     return res
+
+
+def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=1e-2):
+    """Per-cell check of the FP64 path-replay gradient d loss / d log rho on the
+    adjoint workload against common-random-number central differences
+    (eps in log rho).  Cells are drawn with probability proportional to
+    |gradient| (cells no packet reaches have no measurable difference).
+    The packets are split into n_batches id ranges, each an estimate of the
+    same images (source weight n_batches); per batch the FD of that batch's
+    loss is compared with that batch's adjoint, giving mean and standard
+    error over batches."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi, configs
+    from kratos_polrad_amd.engine import Engine, RunSpec
+
+    aw = configs.config4(n_packets=n)
+    w = aw.base
+    eng = Engine(w.mesh)
+    run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+    nb = n // n_batches
+    eng.set_media(aw.kappa_obs, w.albedo, w.g)
+    obs_run = RunSpec(**{**run.__dict__, "packet_offset": 1 << 40})
+    obs = eng.alloc_outputs(obs_run, J=False)
+    eng.forward(w.source, obs_run, obs)
+    torch.cuda.synchronize()
+    q_obs = obs["image"][..., 1:].clone()
+
+    def batch_run(b):
+        r = RunSpec(**{**run.__dict__, "packet_offset": b * nb})
+        r.bands = [type(run.bands[0])(**{**run.bands[0].__dict__, "n_packets": nb})]
+        return r
+
+    def loss(b):
+        out = eng.alloc_outputs(batch_run(b), J=False, counters=False)
+        eng.forward(w.source, batch_run(b), out, weight=float(n_batches))
+        res = out["image"][..., 1:] - q_obs
+        return float((res * res).sum()), out
+
+    adj = []
+    eng.set_media(w.kappa, w.albedo, w.g)
+    for b in range(n_batches):
+        L0, out = loss(b)
+        wgt = torch.zeros_like(out["image"])
+        wgt[..., 1:] = 2.0 * (out["image"][..., 1:] - q_obs)
+        g = eng.adjoint(w.source, batch_run(b), wgt, weight=float(n_batches))
+        adj.append(g["d_logrho"].cpu().numpy())
+    adj = np.array(adj)
+    mean_adj = adj.mean(axis=0)
+    p = np.abs(mean_adj)
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(len(p), size=n_cells, replace=False, p=p / p.sum())
+    def cell_fd(e):
+        fd = np.zeros((n_batches, n_cells))
+        for k, c in enumerate(cells):
+            # the exact log-rho step after rounding kappa_rho to FP32
+            kp = np.float32(w.kappa[c] * np.exp(e))
+            km = np.float32(w.kappa[c] * np.exp(-e))
+            step = float(np.log(np.float64(kp) / np.float64(km)))
+            for sgn, kv in ((1.0, kp), (-1.0, km)):
+                kap = w.kappa.copy()
+                kap[c] = kv
+                eng.set_media(kap, w.albedo, w.g)
+                for b in range(n_batches):
+                    fd[b, k] += sgn * loss(b)[0] / step
+        return fd
+
+    def compare(fd):
+        a = adj[:, cells]
+        m_fd, m_ad = fd.mean(axis=0), a.mean(axis=0)
+        se = (a - fd).std(axis=0, ddof=1) / np.sqrt(n_batches)
+        rel = np.abs(m_ad - m_fd) / np.maximum(np.abs(m_fd), 1e-300)
+        return {"median_rel_diff": float(np.median(rel)),
+                "frac_within_2pct": float(np.mean(rel <= 0.02)),
+                "frac_within_2pct_or_3se": float(np.mean((rel <= 0.02) |
+                                                         (np.abs(m_ad - m_fd) <= 3 * se))),
+                "worst_rel_diff": float(rel.max())}
+
+    t0 = time.perf_counter()
+    per_cell = {f"eps_{eps:g}": compare(cell_fd(eps)), f"eps_{eps_dir:g}": compare(cell_fd(eps_dir))}
+    # directional derivatives d/dt L(log rho + t v) along coherent fields v:
+    # toward the observed medium, a uniform rescaling, and smooth random
+    # k^-11/3 fields; FD vs v . grad
+    xc = w.mesh.cell_centers()
+    dirs = [np.log(aw.kappa_obs.astype(np.float64) / np.maximum(w.kappa, 1e-30)),
+            np.ones(len(p))]
+    for sd in range(6):
+        dirs.append(configs._sample_field(configs.gaussian_random_field(64, 100 + sd), xc))
+    n_dir = len(dirs)
+    dd_fd = np.zeros((n_batches, n_dir))
+    dd_ad = np.zeros((n_batches, n_dir))
+    for k in range(n_dir):
+        v = dirs[k]
+        kp = (w.kappa * np.exp(eps_dir * v)).astype(np.float32)
+        km = (w.kappa * np.exp(-eps_dir * v)).astype(np.float32)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            veff = np.where(w.kappa > 0, np.log(kp.astype(np.float64) / km) / (2 * eps_dir),
+                            0.0)
+        dd_ad[:, k] = adj @ veff
+        for sgn, kv in ((1.0, kp), (-1.0, km)):
+            eng.set_media(kv, w.albedo, w.g)
+            for b in range(n_batches):
+                dd_fd[b, k] += sgn * loss(b)[0] / (2 * eps_dir)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    t_fd = time.perf_counter() - t0
+    dm_fd, dm_ad = dd_fd.mean(axis=0), dd_ad.mean(axis=0)
+    d_se = (dd_ad - dd_fd).std(axis=0, ddof=1) / np.sqrt(n_batches)
+    d_rel = np.abs(dm_ad - dm_fd) / np.abs(dm_fd)
+    return {"workload": w.name, "packets": n, "batches": n_batches, "cells": n_cells,
+            "fd_seconds": t_fd, "per_cell": per_cell,
+            "directional": {"directions": n_dir, "eps": eps_dir,
+                            "rel_diff": [float(x) for x in d_rel],
+                            "diff_over_se": [float(x) for x in np.abs(dm_ad - dm_fd) / d_se],
+                            "max_rel_diff": float(d_rel.max())}}
 
 
 def compaction(n=100_000_000, unit=64, n_bins=2234, p_keep=0.7):
@@ -364,6 +481,8 @@ def main():
 
     if not args.skip_adjoint:
         res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))
+        if args.fd_cells > 0:
+            res["adjoint"]["fd_check"] = adjoint_fd(int(args.adjoint_packets), args.fd_cells)
 
     if not args.skip_compaction:
         res["compaction"] = compaction()
