@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
     p.add_argument("--skip-compaction", action="store_true")
+    p.add_argument("--skip-large", action="store_true")
+    p.add_argument("--large-packets", type=float, default=1e9)
     p.add_argument("--fd-cells", type=int, default=64,
                    help="cells of the per-cell finite-difference check of the adjoint (0: off)")
     p.add_argument("--batched-scale", type=float, default=1.0,
@@ -308,6 +310,71 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
                             "max_rel_diff": float(d_rel.max())}}
 
 
+def large(packets=1_000_000_000, chunk=100_000_000):
+    """Config 5: 1e9 packets on the ~28,000-block disk mesh, streamed as device
+    chunks of `chunk` packets replayed from one hipGraph (autotuned schedule,
+    mixed precision); FP64 on one chunk for the ratio; one path-replay pass
+    (mixed) for d(total I flux)/d log rho over one chunk."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi, configs
+    from kratos_polrad_amd.engine import Engine, RunSpec
+
+    t0 = time.perf_counter()
+    w = configs.config5(n_packets=chunk)
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    t_setup = time.perf_counter() - t0
+    res = {"workload": w.name, "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
+           "levels": int(w.mesh.level.max()) + 1, "packets": packets, "chunk": chunk,
+           "n_incl": len(w.run.incl_deg), "image_n": w.run.image_n, "setup_s": t_setup}
+    res["schedule"] = eng.autotune(w.source, w.run, sample_packets=4_000_000)
+    n_chunks = packets // chunk
+    out = eng.alloc_outputs(w.run)
+    g = eng.graph(w.source, w.run, out, n_chunks=n_chunks)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.launch()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cnt = out["counters"].cpu().numpy().astype(np.float64)
+    res["mixed"] = {"seconds": wall, "value": float((cnt[0] + cnt[1]) / wall),
+                    "packets_done": float(cnt[7]), "crossings": float(cnt[0] + cnt[1])}
+    img32 = out["image"].cpu().numpy()
+    del g, out
+    # FP64 and mixed on the same first chunk
+    runs = {}
+    for prec in (_abi.KP_F64, _abi.KP_F32):
+        run = RunSpec(**{**w.run.__dict__, "precision": prec})
+        o = eng.alloc_outputs(run)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.forward(w.source, run, o)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        c = o["counters"].cpu().numpy().astype(np.float64)
+        runs[prec] = (dt, (c[0] + c[1]) / dt, o["image"].cpu().numpy())
+        del o
+    res["fp64_one_chunk"] = {"seconds": runs[_abi.KP_F64][0], "value": runs[_abi.KP_F64][1]}
+    res["mixed_one_chunk"] = {"seconds": runs[_abi.KP_F32][0], "value": runs[_abi.KP_F32][1]}
+    res["speedup_mixed_vs_fp64"] = runs[_abi.KP_F32][1] / runs[_abi.KP_F64][1]
+    res["image_error_mixed_vs_fp64_same_packets"] = [
+        img_err(runs[_abi.KP_F32][2][0, o], runs[_abi.KP_F64][2][0, o])
+        for o in range(len(w.run.incl_deg))]
+    # one path-replay pass: gradient of the total I flux of all images
+    wgt = torch.zeros(img32.shape, dtype=torch.float64, device="cuda")
+    wgt[..., 0] = 1.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gr = eng.adjoint(w.source, w.run, wgt)
+    torch.cuda.synchronize()
+    res["gradient_pass"] = {"seconds": time.perf_counter() - t0, "packets": chunk,
+                            "d_logrho_norm": float(torch.linalg.vector_norm(gr["d_logrho"])),
+                            "jlin": float(gr["jlin"][0])}
+    return res
+
+
 def compaction(n=100_000_000, unit=64, n_bins=2234, p_keep=0.7):
     """Pool compaction and per-destination binning + gather of n packet
     records of `unit` bytes (device-resident inputs), timed with HIP events
@@ -486,6 +553,9 @@ def main():
 
     if not args.skip_compaction:
         res["compaction"] = compaction()
+
+    if not args.skip_large:
+        res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000))
 
     if not args.no_cpu:
         import oracle

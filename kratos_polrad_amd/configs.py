@@ -16,6 +16,15 @@ log-spaced over 0.3-3 um: extinction scaled by (lambda/0.55 um)^-1.7, HG g
 falling linearly from 0.7 to 0.3 across the bands, albedo 0.5, uneven packet
 counts drawn log-uniform in [1e5, 1e7] from the seed; all bands in one launch.
 
+Config 5 (large): a flared disk rho ~ (r/r0)^-1 exp(-z^2 / 2h^2), h = 0.1 r^1.25
+(cylindrical r from the domain axis, unit-cube units, r0 = 0.1), an inner
+hole r < 0.05, times a seeded lognormal field (sigma 0.3); 5-level block AMR
+on an 8^3 root grid of 16^3 blocks (128^3 root cells), refined where
+max |grad rho| dx / (rho + 2e-3) > 0.2 (~28,000 blocks, 1.2e8 cells); HG
+g = 0.5, albedo 0.6, p_max 0.5; the central source; 256^2 I/Q/U images at
+three inclinations; kappa_rho normalised to a midplane optical depth of 20
+from the inner rim to the edge.
+
 Config 4 (adjoint): the config-2 field on a 2-level mesh of 3^3 root blocks
 (96^3 effective); the "observed" Q/U image is rendered from the same field
 times an extra seeded lognormal factor (sigma 0.2); loss = sum of squared Q
@@ -233,3 +242,140 @@ def config4(n_packets: int = 10_000_000, seed: int = 42, tau_centre: float = 5.0
                   precision=_abi.KP_F64, incl_deg=(90.0,), image_n=128, image_half=0.5)
     base = Workload("config4_amr2_96_adjoint", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
     return AdjointWorkload(base, kappa_obs)
+
+
+def disk_rho(x: np.ndarray, grf: np.ndarray, sigma: float = 0.3, r_in: float = 0.05,
+             r0: float = 0.1, h0: float = 0.1, flare: float = 1.25) -> np.ndarray:
+    """Flared-disk density at points x [m,3] of the unit cube (centre 0.5)."""
+    c = x - 0.5
+    r = np.sqrt(c[:, 0] ** 2 + c[:, 1] ** 2)
+    rr = np.maximum(r, 1e-6)
+    h = h0 * rr ** flare
+    ln = sigma * _sample_field(grf, x) - 0.5 * sigma ** 2
+    rho = (rr / r0) ** -1 * np.exp(-c[:, 2] ** 2 / (2 * h * h) + ln)
+    rho[r < r_in] = 0.0
+    return rho
+
+
+def disk_mesh(grf: np.ndarray, root: int = 8, n_cell: int = 16, levels: int = 5,
+              threshold: float = 0.2, floor: float = 2e-3, samples: int = 8) -> Mesh:
+    """Criterion-refined block mesh of the disk: a block at level l is refined
+    when max |grad rho| (cell width) / (rho + floor) over samples^3 points of
+    the block exceeds threshold (2:1 balance kept by the tree)."""
+    refine = {}
+    cur = [(i, j, k) for i in range(root) for j in range(root) for k in range(root)]
+    for lvl in range(levels - 1):
+        bs = 1.0 / (root << lvl)
+        c = (np.arange(samples) + 0.5) / samples * bs
+        gx, gy, gz = np.meshgrid(c, c, c, indexing="ij")
+        off = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
+        blk = np.array(cur, dtype=np.float64) * bs
+        nxt = []
+        for s0 in range(0, len(blk), 2000):
+            B = blk[s0:s0 + 2000]
+            pts = (B[:, None, :] + off[None]).reshape(-1, 3)
+            rho = disk_rho(pts, grf).reshape(len(B), samples, samples, samples)
+            gr = np.sqrt(sum(np.gradient(rho, axis=a) ** 2 for a in (1, 2, 3)))
+            crit = (gr * (samples / n_cell) / (rho + floor)).reshape(len(B), -1).max(axis=1)
+            for q in np.nonzero(crit > threshold)[0]:
+                i, j, k = cur[s0 + q]
+                refine.setdefault(lvl + 1, []).append((i, j, k))
+                nxt += [(2 * i + a, 2 * j + b, 2 * k + d)
+                        for a in range(2) for b in range(2) for d in range(2)]
+        cur = nxt
+    mesh = Mesh(root_blocks=(root, root, root), n_cell=n_cell)
+    for lvl in sorted(refine):
+        kids = [(2 * i + a, 2 * j + b, 2 * k + d) for (i, j, k) in refine[lvl]
+                for a in range(2) for b in range(2) for d in range(2)]
+        mesh.refine(lvl, np.array(kids, dtype=np.int32))
+    return mesh.finalize()
+
+
+def cell_field(mesh: Mesh, f, chunk_blocks: int = 2048) -> np.ndarray:
+    """f evaluated at every cell centre, block chunk by block chunk (FP32)."""
+    n = mesh.n_cell
+    c = np.arange(n) + 0.5
+    gi, gj, gk = np.meshgrid(c, c, c, indexing="ij")
+    # cell order i0 + n*(i1 + n*i2): i0 fastest
+    off = np.stack([gi.transpose(2, 1, 0).ravel(), gj.transpose(2, 1, 0).ravel(),
+                    gk.transpose(2, 1, 0).ravel()], 1)
+    out = np.empty(mesh.n_cells, dtype=np.float32)
+    n3 = n ** 3
+    for b0 in range(0, mesh.n_blocks, chunk_blocks):
+        b1 = min(b0 + chunk_blocks, mesh.n_blocks)
+        pts = (mesh.xf0[b0:b1, None, :] + off[None] * mesh.dx0[b0:b1, None, :]).reshape(-1, 3)
+        out[b0 * n3:b1 * n3] = f(pts)
+    return out
+
+
+def _disk_rho_torch(x, grf_t, sigma=0.3, r_in=0.05, r0=0.1, h0=0.1, flare=1.25):
+    """disk_rho on a torch device (FP64): periodic trilinear field lookup."""
+    import torch
+
+    n = grf_t.shape[0]
+    u = x * n - 0.5
+    i0 = torch.floor(u)
+    f = u - i0
+    i0 = i0.long()
+    acc = torch.zeros(x.shape[0], dtype=torch.float64, device=x.device)
+    for dx in (0, 1):
+        wx = f[:, 0] if dx else 1 - f[:, 0]
+        ix = (i0[:, 0] + dx) % n
+        for dy in (0, 1):
+            wy = f[:, 1] if dy else 1 - f[:, 1]
+            iy = (i0[:, 1] + dy) % n
+            for dz in (0, 1):
+                wz = f[:, 2] if dz else 1 - f[:, 2]
+                iz = (i0[:, 2] + dz) % n
+                acc += wx * wy * wz * grf_t[ix, iy, iz]
+    c = x - 0.5
+    r = torch.sqrt(c[:, 0] ** 2 + c[:, 1] ** 2)
+    rr = torch.clamp(r, min=1e-6)
+    h = h0 * rr ** flare
+    rho = (rr / r0) ** -1 * torch.exp(-c[:, 2] ** 2 / (2 * h * h) + sigma * acc - 0.5 * sigma ** 2)
+    return torch.where(r < r_in, torch.zeros_like(rho), rho)
+
+
+def cell_field_disk(mesh: Mesh, grf: np.ndarray, scale: float) -> np.ndarray:
+    """scale * disk_rho at every cell centre; on the GPU when torch sees one."""
+    try:
+        import torch
+
+        use_gpu = torch.cuda.is_available()
+    except ImportError:
+        use_gpu = False
+    if not use_gpu:
+        return cell_field(mesh, lambda x: disk_rho(x, grf) * scale)
+    dev = torch.device("cuda")
+    grf_t = torch.from_numpy(grf).to(dev)
+    n = mesh.n_cell
+    c = torch.arange(n, dtype=torch.float64, device=dev) + 0.5
+    g2, g1, g0 = torch.meshgrid(c, c, c, indexing="ij")  # i0 fastest in the flattening
+    off = torch.stack([g0.reshape(-1), g1.reshape(-1), g2.reshape(-1)], 1)
+    xf0 = torch.from_numpy(np.ascontiguousarray(mesh.xf0)).to(dev)
+    dx0 = torch.from_numpy(np.ascontiguousarray(mesh.dx0)).to(dev)
+    out = np.empty(mesh.n_cells, dtype=np.float32)
+    n3 = n ** 3
+    for b0 in range(0, mesh.n_blocks, 1024):
+        b1 = min(b0 + 1024, mesh.n_blocks)
+        pts = (xf0[b0:b1, None, :] + off[None] * dx0[b0:b1, None, :]).reshape(-1, 3)
+        out[b0 * n3:b1 * n3] = (_disk_rho_torch(pts, grf_t) * scale).float().cpu().numpy()
+    return out
+
+
+def config5(n_packets: int = 1_000_000_000, precision: int = _abi.KP_F32, seed: int = 42,
+            tau_mid: float = 20.0, field_seed: int = 21) -> Workload:
+    grf = gaussian_random_field(112, field_seed)
+    mesh = disk_mesh(grf)
+    # midplane column from the inner rim to the edge along +x
+    t = 0.05 + (np.arange(20000) + 0.5) / 20000 * 0.45
+    line = np.stack([0.5 + t, np.full_like(t, 0.5), np.full_like(t, 0.5)], 1)
+    col = float(np.sum(disk_rho(line, grf)) * 0.45 / 20000)
+    k0 = tau_mid / col
+    kappa = cell_field_disk(mesh, grf, k0)
+    albedo = np.full(mesh.n_cells, 0.6, dtype=np.float32)
+    g = np.full(mesh.n_cells, 0.5, dtype=np.float32)
+    run = RunSpec(bands=[BandSpec(n_packets=n_packets, albedo=0.6, g=0.5)], seed=seed,
+                  scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
+                  incl_deg=(30.0, 60.0, 85.0), image_n=256, image_half=0.5)
+    return Workload("config5_amr5_disk_1e9", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
