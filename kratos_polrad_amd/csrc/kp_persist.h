@@ -710,58 +710,38 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         if (is_main) {
             L.ncross += 1;
             L.steps -= 1;
+            // the segment ends at the exit face, or at the interaction point
+            // when the remaining optical depth runs out inside the cell
+            bool hit;
+            real seg;
             if constexpr (sizeof(real) == 8) {
-                const double V = P.vol0[L.cur.ib];
-                if (kap > 0.0 && dtau >= L.tau) {
-                    double li = L.tau / kap;
-                    if (li > l)
-                        li = l;
-                    if constexpr (ADJ) {
-                        if (L.pass == 2 && P.g_logrho && L.B != 0.0)
-                            atomic_add_f64(g_base(P, P.g_logrho) + gc, -((kap * li) * L.B));
-                    } else {
-                        if (P.J)
-                            j_deposit(P, lds, gc, L.band, L.I * (li / V));
-                    }
-                    Pol::advance(L.cur, li);
-                    L.gc = gc;
-                    L.mode = M_PEND;
-                    L.pa = PA_INTERACT;
-                    continue;
-                }
-                L.tau = L.tau - dtau;
-                if constexpr (ADJ) {
-                    if (L.pass == 2 && P.g_logrho && L.B != 0.0)
-                        atomic_add_f64(g_base(P, P.g_logrho) + gc, -(dtau * L.B));
-                } else {
-                    if (P.J)
-                        j_deposit(P, lds, gc, L.band, L.I * (l / V));
+                hit = kap > 0.0 && dtau >= L.tau;
+                seg = l;
+                if (hit) {
+                    const double li = L.tau / kap;
+                    seg = li > l ? l : li;
                 }
             } else {
-                const float iV = L.cur.iv;
-                if (kap > 0.0f && (double)dtau >= L.tau) {
-                    const float li = fminf((float)L.tau * f32::fast_rcp(kap), l);
-                    if constexpr (ADJ) {
-                        if (L.pass == 2 && P.g_logrho && L.B != 0.0)
-                            atomic_add_f64(g_base(P, P.g_logrho) + gc,
-                                           -((double)(kap * li) * L.B));
-                    } else {
-                        L.acc.add(P, L.band, gc, L.I * li * iV, lds);
-                    }
-                    Pol::advance(L.cur, li);
-                    L.gc = gc;
-                    L.mode = M_PEND;
-                    L.pa = PA_INTERACT;
-                    continue;
-                }
-                L.tau -= (double)dtau;
-                if constexpr (ADJ) {
-                    if (L.pass == 2 && P.g_logrho && L.B != 0.0)
-                        atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.B));
-                } else {
-                    L.acc.add(P, L.band, gc, L.I * l * iV, lds);
-                }
+                hit = kap > 0.0f && (double)dtau >= L.tau;
+                seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
             }
+            if constexpr (ADJ) {
+                if (L.pass == 2 && P.g_logrho && L.B != 0.0)
+                    atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)(kap * seg) * L.B));
+            } else if constexpr (sizeof(real) == 8) {
+                if (P.J)
+                    j_deposit(P, lds, gc, L.band, L.I * (seg / P.vol0[L.cur.ib]));
+            } else {
+                L.acc.add(P, L.band, gc, L.I * seg * L.cur.iv, lds);
+            }
+            if (hit) {
+                Pol::advance(L.cur, seg);
+                L.gc = gc;
+                L.mode = M_PEND;
+                L.pa = PA_INTERACT;
+                continue;
+            }
+            L.tau = L.tau - (double)dtau;
         } else {
             L.tau = L.tau + (double)dtau;
             L.npeel += 1;
