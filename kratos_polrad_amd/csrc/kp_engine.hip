@@ -415,13 +415,13 @@ void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t 
 // line fetch + write-back.  n_live is the part of one replica that the
 // packets in flight update (one band's slice when bands are laid out
 // band-major: slots are claimed in band order); the whole set of replicas
-// is capped at 4 GiB.
+// is capped at 32 GiB (of the 288 GB of HBM).
 uint32_t pick_replicas(int requested, uint64_t n, uint64_t n_live, uint64_t blocks,
                        uint64_t budget_bytes)
 {
     uint64_t R =
         requested > 0 ? (uint64_t)requested : budget_bytes / (n_live * sizeof(double) + 1);
-    const uint64_t hard = (4ull << 30) / (n * sizeof(double) + 1);
+    const uint64_t hard = (32ull << 30) / (n * sizeof(double) + 1);
     if (R > hard)
         R = hard;
     if (R > blocks)

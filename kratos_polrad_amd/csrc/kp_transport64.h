@@ -147,11 +147,14 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, do
         }
         reload = true;
     }
+    // periodic wrap (par_base_t::regulate); a ray that just left through the
+    // lower domain face sits exactly on it and belongs to the upper image
 #pragma unroll
     for (int q = 0; q < 3; ++q)
         if (P.periodic[q]) {
             const double lbox = P.xmax[q] - P.xmin[q];
-            if (r.x[q] < P.xmin[q]) {
+            const bool down_exit = (q == a) && !(r.d[q] > 0.0);
+            if (r.x[q] < P.xmin[q] || (down_exit && r.x[q] == P.xmin[q])) {
                 r.x[q] += lbox;
                 reload = true;
             } else if (r.x[q] >= P.xmax[q]) {

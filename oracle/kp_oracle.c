@@ -161,10 +161,14 @@ static int cross_face(const kp_mesh_desc *m, ray_t *r, int ap, double l, int *re
         *reload = 1;
         break;
     }
+    /* periodic wrap (particle_base.h:37-56); deviation (3): a ray that just
+     * left through the lower domain face sits exactly on it and is moved to
+     * the upper image (the reference's strict x < x_min leaves it at x_min) */
     for (int a = 0; a < 3; ++a)
         if (m->periodic[a]) {
             const double lbox = m->x_max[a] - m->x_min[a];
-            if (r->x[a] < m->x_min[a]) {
+            const int down_exit = (a == ap) && !(r->d[a] > 0.0);
+            if (r->x[a] < m->x_min[a] || (down_exit && r->x[a] == m->x_min[a])) {
                 r->x[a] += lbox;
                 *reload = 1;
             } else if (r->x[a] >= m->x_max[a]) {

@@ -18,14 +18,18 @@
  *    Mueller scattering with Stokes frame rotation (Chandrasekhar 1960),
  *    peel-off images (Yusef-Zadeh, Morris & White 1984) and the path-length
  *    mean-intensity tally (Lucy 1999).
- * Deviations from the reference march, both inert on the dyadic meshes the
- * reference produces for pure streaming:
+ * Deviations from the reference march (the first two inert on the dyadic
+ * meshes the reference produces for pure streaming):
  *  (1) after a coarse->fine face crossing the tangential fine cell is the one
  *      containing the position (the reference always takes the lower one,
  *      proxy.h:383-389, and geo_loc_t::fix, particle_base.h:119-127, does
  *      not update the index);
  *  (2) a negative face distance on a moving axis (position one ulp past a
- *      face after an interaction) is clamped to zero.
+ *      face after an interaction) is clamped to zero;
+ *  (3) a ray leaving through the lower face of a periodic axis is moved to
+ *      the upper periodic image at once (the reference's strict
+ *      x < x_min test in regulate leaves it on x_min, inside the upper
+ *      block's index range but outside its face bounds, for one step).
  */
 #ifndef KP_ORACLE_H
 #define KP_ORACLE_H

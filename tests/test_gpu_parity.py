@@ -146,3 +146,42 @@ def test_persistent_refill_processes_each_packet_once(chunk, bt, batch):
                                  blocks_per_cu=2, chunk=chunk, event_batch=batch))
     assert int(got["counters"][_abi.KP_C_PACKETS]) == 3000
     np.testing.assert_array_equal(got["end_state"], ref["end_state"])
+
+
+def _periodic_workload(n_packets, precision=_abi.KP_F64):
+    """Periodic in x and y, refined off-centre so packets cross level jumps
+    and the periodic faces; J only (images need a bounded domain)."""
+    mesh = Mesh(root_blocks=(3, 2, 2), n_cell=8, x_max=(1.5, 1.0, 1.0), periodic=(1, 1, 0))
+    mesh.refine_region(1, (1.2, 0.0, 0.3), (1.5, 0.4, 0.7))
+    mesh.finalize()
+    xc = mesh.cell_centers()
+    kappa = (1.5 + 0.6 * np.sin(7 * xc[:, 0]) + 0.6 * np.cos(5 * xc[:, 1])).astype(np.float32)
+    albedo = np.full(mesh.n_cells, 0.9, dtype=np.float32)
+    g = np.full(mesh.n_cells, 0.4, dtype=np.float32)
+    run = RunSpec(bands=[BandSpec(n_packets=n_packets, albedo=0.9, g=0.4)], seed=77,
+                  scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
+                  step_max=5000, incl_deg=())
+    return configs.Workload("periodic_test", mesh, kappa, albedo, g, (1.37, 0.05, 0.5), run)
+
+
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+def test_f64_periodic_bit_exact(oracle_lib, sched):
+    w = _periodic_workload(4000)
+    got = _run_gpu(w, sched=sched)
+    ref = oracle_lib.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, w.run, end_state=True)
+    _compare_end_states(got["end_state"], ref["end_state"])
+    assert np.array_equal(got["counters"], ref["counters"])
+    # packets wrap: some end far outside the refined box, none are lost
+    assert int(got["counters"][_abi.KP_C_LOST]) == 0
+    np.testing.assert_allclose(got["J"], ref["J"], rtol=1e-12, atol=1e-12 * ref["J"].max())
+
+
+def test_f32_periodic_tracks_f64():
+    w64 = _periodic_workload(20000)
+    w32 = _periodic_workload(20000, precision=_abi.KP_F32)
+    a = _run_gpu(w64, end_state=False)
+    b = _run_gpu(w32, end_state=False)
+    assert int(b["counters"][_abi.KP_C_PACKETS]) == 20000
+    assert int(b["counters"][_abi.KP_C_LOST]) == 0
+    # same random numbers: totals agree far inside the Monte Carlo noise
+    assert abs(b["J"].sum() / a["J"].sum() - 1) < 2e-3

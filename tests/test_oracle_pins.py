@@ -266,6 +266,34 @@ def test_periodic_wrap(oracle_lib):
     assert 0.0 <= xe[0] < 1.0
 
 
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_periodic_wrap_both_directions_no_zero_steps(oracle_lib, sign):
+    """A ray crossing the periodic faces in either direction visits every
+    cell on its line exactly as in the unwrapped domain: step lengths sum to
+    the travelled distance, no zero-length steps, cells visited match point
+    location at each step midpoint (lower-face crossings included)."""
+    m = Mesh(root_blocks=(2, 2, 1), n_cell=4, periodic=(1, 1, 0))
+    m.refine(1, [[0, 0, 0], [3, 3, 1]])
+    m.finalize()
+    x = (0.31, 0.62, 0.13)
+    d = np.array([sign * 0.8, sign * -0.5, 0.33])
+    d /= np.linalg.norm(d)
+    ib, cell, ln, xe, st = oracle_lib.trace(m, x, tuple(d))
+    assert st == 1  # leaves through z = 1
+    assert np.all(ln > 0)
+    assert abs(ln.sum() - (1.0 - 0.13) / d[2]) < 1e-12
+    # every step's midpoint lies in the cell the march assigned
+    pos = np.array(x, dtype=np.float64)
+    for k in range(len(ln)):
+        mid = pos + 0.5 * ln[k] * d
+        w = mid.copy()
+        for a in range(2):
+            w[a] %= 1.0
+        b, i = m.locate(w)
+        assert (b, i[0] + 4 * (i[1] + 4 * i[2])) == (ib[k], cell[k]), k
+        pos = pos + ln[k] * d
+
+
 # --------------------------------------------------------------------------
 # Transport: closed forms
 def _uniform_media(m, kappa):
