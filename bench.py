@@ -158,7 +158,7 @@ def batched(steps, scale):
         run = RunSpec(**{**w.run.__dict__, "precision": prec, "packet_offset": offset})
         # each precision runs with its own tuned schedule (cached per precision)
         t0 = time.perf_counter()
-        sched = eng.autotune(w.source, run, sample_packets=2_000_000)
+        sched = eng.autotune(w.source, run, sample_packets=10_000_000)
         res["autotune_s"] += time.perf_counter() - t0
         res["schedule_mixed" if prec == _abi.KP_F32 else "schedule_fp64"] = sched
         out = eng.alloc_outputs(run)
@@ -481,7 +481,7 @@ def main():
     def timed(prec, steps, warmup, offset0=0):
         run = RunSpec(**{**w.run.__dict__, "precision": prec})
         # each precision runs with its own tuned schedule (cached per precision)
-        eng.autotune(w.source, run, sample_packets=2_000_000)
+        eng.autotune(w.source, run, sample_packets=5_000_000)
         out = eng.alloc_outputs(run)
         for k in range(warmup):
             run.packet_offset = (1 << 50) + k * n

@@ -1072,7 +1072,7 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                     }
                 }
             };
-            try_knob(&kp_schedule::j_replicas, {0, 1, 2, 4, 16});
+            try_knob(&kp_schedule::j_replicas, {0, 1, 2, 3, 4, 8});
             try_knob(&kp_schedule::img_replicas, {0, 16, 64});
             try_knob(&kp_schedule::event_batch, {4, 8, 16});
             if (run->precision == KP_F64)
