@@ -22,6 +22,10 @@
  *                                        types/proxy.h:64-121
  *   pool compaction / send gather ...... types/particle_pool.h:191-248,
  *                                        boundary/comm.h:118-127,226-239
+ *   spherical coordinates (mesh and
+ *     photon march) .................... mesh/meshgen/spherical.h:9-140,
+ *                                        mesh/meshgen/cylindrical.h:18-43,
+ *                                        radiation/photon_sph.h:10-266
  */
 #ifndef KPOLRAD_H
 #define KPOLRAD_H
@@ -51,6 +55,20 @@ const char *kp_last_error(void);
 
 typedef struct kp_mesh kp_mesh;
 
+/* Coordinate system of a mesh: the meshgen class the reference's problem
+ * code instantiates (meshgen::base_t or meshgen::sph_t). */
+enum { KP_COORD_CARTESIAN = 0, KP_COORD_SPHERICAL = 1 };
+
+/* Per-block geometry table of a spherical mesh (doubles), with m = n_cell+1:
+ *   [0,m)         r of the r faces            (geo_t::x_fc axis 0)
+ *   [m,2m)        cos(theta) of the theta faces
+ *   [2m,3m)       sin(phi) of the phi faces
+ *   [3m,4m)       cos(phi) of the phi faces
+ *   [4m,4m+n)     (r_{k+1}^3 - r_k^3)/3       (sph_t::volume factors,
+ *   [4m+n,4m+2n)  |cos th_k - cos th_{k+1}|    spherical.h:31-42; a cell's
+ *   [4m+2n,4m+3n) phi_{k+1} - phi_k            volume is (f_r*f_th)*f_phi)  */
+#define KP_SPH_TAB(n_cell) (4 * ((n_cell) + 1) + 3 * (n_cell))
+
 /* Block-table view of a mesh (arrays owned by the kp_mesh).  Block b has
  * cells i in [0,n_cell)^3; its cell (i0,i1,i2) has linear index
  *   b*n_cell^3 + i0 + n_cell*(i1 + n_cell*i2)
@@ -74,6 +92,10 @@ typedef struct kp_mesh_desc {
     const uint8_t *nb_shift; /* [n_blocks][24]                             */
     const int8_t *d_lvl;   /* [n_blocks][6]  neighbour level - own level   */
     const uint8_t *phybnd; /* [n_blocks] bit 2a+s: physical boundary       */
+    int32_t coord;         /* KP_COORD_*; spherical axes are (r,theta,phi) */
+    int32_t pad_;
+    const double *xf;      /* [n_blocks][3][n_cell+1] face coordinates     */
+    const double *sph;     /* [n_blocks][KP_SPH_TAB(n_cell)] or NULL       */
 } kp_mesh_desc;
 
 /* meshgen::base_t::read/init (meshgen.cpp:13-106): root grid of
@@ -82,6 +104,16 @@ kp_status kp_mesh_create(const int32_t root_blocks[3], int32_t n_cell,
                          const double x_min[3], const double x_max[3],
                          const int32_t periodic[3], kp_mesh **out);
 void kp_mesh_destroy(kp_mesh *mesh);
+
+/* Coordinate system, before kp_mesh_finalize.  KP_COORD_SPHERICAL makes the
+ * axes (r, theta, phi) (meshgen::sph_t, spherical.h:9-140): x_min/x_max of
+ * kp_mesh_create are the r, theta and phi limits; 0 < r_min, 0 <= theta_min
+ * < theta_max <= pi, phi width <= 2 pi (periodic only when it is 2 pi) and
+ * every phi cell narrower than pi.  r_ratio is [mesh] r_ratio of
+ * cyl_t::read (cylindrical.h:18-32): 1 = uniform r, > 0 = ratio of
+ * consecutive root-level r widths, < 0 = logarithmic spacing
+ * (r_max/r_min)^(1/n_r). */
+kp_status kp_mesh_set_coord(kp_mesh *mesh, int32_t coord, double r_ratio);
 
 /* tree_t::insert (tree.cpp:202-218) of n logical regions at `level`
  * (idx is [n][3]); parents are inserted and masked, 2:1 balance kept. */
@@ -100,7 +132,8 @@ kp_status kp_mesh_finalize(kp_mesh *mesh);
 kp_status kp_mesh_describe(const kp_mesh *mesh, kp_mesh_desc *desc);
 
 /* Leaf block and cell containing x (cells are half-open [x_l,x_r)),
- * geo_t::idx semantics (geometry.h:120-140).  ib = -1 if outside. */
+ * geo_t::idx semantics (geometry.h:120-140), x in mesh coordinates
+ * ((r, theta, phi) on a spherical mesh).  ib = -1 if outside. */
 kp_status kp_mesh_locate(const kp_mesh *mesh, const double x[3],
                          int32_t *ib, int32_t i[3]);
 
@@ -225,7 +258,12 @@ kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);
 kp_status kp_get_schedule(const kp_engine *eng, kp_schedule *sched);
 
 /* Emit, march, scatter and peel off every packet of every band in one
- * launch on `stream` (asynchronous). */
+ * launch on `stream` (asynchronous).
+ * On an engine made from a spherical mesh this is the reference's spherical
+ * photon (photon_sph.h:10-266 driven by photon.h:104-125): pure streaming
+ * from the (Cartesian) source point with the path-length tally, in FP64;
+ * kappa_scale must be 0, n_incl 0 and precision KP_F64, and the medium is not
+ * needed.  end_state, counters, graphs and both schedules work as above. */
 kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,
                      kp_tallies *out, void *stream);
 

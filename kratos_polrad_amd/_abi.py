@@ -26,6 +26,7 @@ COUNTER_NAMES = ["cross_main", "cross_peel", "events", "escaped", "killed_rr",
 KP_ES_X, KP_ES_D, KP_ES_I, KP_ES_Q, KP_ES_U = 0, 3, 6, 7, 8
 KP_ES_EVENTS, KP_ES_CROSS, KP_ES_STATUS, KP_ES_IB, KP_ES_CELL, KP_ES_STRIDE = 9, 10, 11, 12, 13, 16
 KP_END_ESCAPED, KP_END_RR, KP_END_STEPS, KP_END_LOST = 1, 2, 3, 4
+KP_COORD_CARTESIAN, KP_COORD_SPHERICAL = 0, 1
 
 
 class MeshDesc(C.Structure):
@@ -44,7 +45,16 @@ class MeshDesc(C.Structure):
         ("nb_shift", C.POINTER(C.c_uint8)),
         ("d_lvl", C.POINTER(C.c_int8)),
         ("phybnd", C.POINTER(C.c_uint8)),
+        ("coord", C.c_int32),
+        ("pad_", C.c_int32),
+        ("xf", C.POINTER(C.c_double)),
+        ("sph", C.POINTER(C.c_double)),
     ]
+
+
+def sph_tab_len(n_cell: int) -> int:
+    """KP_SPH_TAB: doubles per block of a spherical mesh's geometry table."""
+    return 4 * (n_cell + 1) + 3 * n_cell
 
 
 class Band(C.Structure):
@@ -138,6 +148,7 @@ SIGNATURES = {
     "kp_mesh_create": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(_P)]),
     "kp_mesh_destroy": (None, [_P]),
+    "kp_mesh_set_coord": (C.c_int, [_P, C.c_int32, C.c_double]),
     "kp_mesh_refine": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.c_int64]),
     "kp_mesh_refine_region": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),

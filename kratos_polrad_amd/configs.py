@@ -25,6 +25,12 @@ g = 0.5, albedo 0.6, p_max 0.5; the central source; 256^2 I/Q/U images at
 three inclinations; kappa_rho normalised to a midplane optical depth of 20
 from the inner rim to the edge.
 
+Spherical (the reference's spherical photon, photon_sph.h, on meshgen::sph_t):
+a shell r in [0.1, 1] with logarithmic r spacing, the full theta range and a
+periodic phi axis, 8x8x16 root blocks of 16^3 cells, two levels of static
+refinement around an off-centre point source at (0.25, 0.15, 0.1); pure
+streaming with the path-length tally.
+
 Config 4 (adjoint): the config-2 field on a 2-level mesh of 3^3 root blocks
 (96^3 effective); the "observed" Q/U image is rendered from the same field
 times an extra seeded lognormal factor (sigma 0.2); loss = sum of squared Q
@@ -379,3 +385,33 @@ def config5(n_packets: int = 1_000_000_000, precision: int = _abi.KP_F32, seed: 
                   scatter_model=_abi.KP_HG_RAYLEIGH, p_max=0.5, precision=precision,
                   incl_deg=(30.0, 60.0, 85.0), image_n=256, image_half=0.5)
     return Workload("config5_amr5_disk_1e9", mesh, kappa, albedo, g, (0.5, 0.5, 0.5), run)
+
+
+def sph_mesh(root_blocks=(4, 4, 8), n_cell: int = 16, r_lim=(0.1, 1.0), r_ratio: float = -1.0,
+             theta=(0.0, np.pi), phi=(0.0, 2.0 * np.pi), refine=()) -> Mesh:
+    """Spherical block mesh (axes r, theta, phi); ``refine`` is a sequence of
+    (level, lo, hi) boxes in (r, theta, phi).  phi is periodic when it spans
+    2 pi."""
+    per = (0, 0, 1 if abs(phi[1] - phi[0] - 2.0 * np.pi) < 1e-12 else 0)
+    mesh = Mesh(root_blocks=root_blocks, n_cell=n_cell, x_min=(r_lim[0], theta[0], phi[0]),
+                x_max=(r_lim[1], theta[1], phi[1]), periodic=per, coord="spherical",
+                r_ratio=r_ratio)
+    for lvl, lo, hi in refine:
+        mesh.refine_region(lvl, lo, hi)
+    return mesh.finalize()
+
+
+SPH_SOURCE = (0.25, 0.15, 0.1)
+
+
+def config_sph(n_packets: int = 100_000_000, seed: int = 42, n_bands: int = 1,
+               root_blocks=(8, 8, 16)) -> Workload:
+    """Spherical streaming workload (see the module docstring)."""
+    mesh = sph_mesh(root_blocks=root_blocks,
+                    refine=((1, (0.2, 0.9, 0.2), (0.45, 1.6, 0.9)),
+                            (2, (0.26, 1.1, 0.4), (0.36, 1.4, 0.7))))
+    per_band = max(1, int(n_packets) // n_bands)
+    run = RunSpec(bands=[BandSpec(n_packets=per_band, kappa_scale=0.0) for _ in range(n_bands)],
+                  seed=seed, precision=_abi.KP_F64, step_max=100000)
+    z = np.zeros(0, dtype=np.float32)
+    return Workload("sph_shell_amr2_stream", mesh, z, z, z, SPH_SOURCE, run)

@@ -34,6 +34,8 @@ struct kparams {
     const float *h32;
     const float *org32;
     const float *vol32;
+    // spherical mesh: per-block face/volume table (KP_SPH_TAB doubles), else null
+    const double *sph;
     int n_cell;
     int n3;
     int periodic[3];

@@ -21,6 +21,7 @@
 #include "kp_error.h"
 #include "kp_adjoint.h"
 #include "kp_persist.h"
+#include "kp_sph.h"
 
 #define KP_HIP(call)                                                                        \
     do {                                                                                    \
@@ -51,6 +52,10 @@ struct kp_engine {
     uint8_t *nb_shift = nullptr, *phybnd = nullptr;
     int8_t *d_lvl = nullptr;
     float *h32 = nullptr, *org32 = nullptr, *vol32 = nullptr;
+    // spherical mesh: per-block face/volume table (device + host copy)
+    int coord = KP_COORD_CARTESIAN;
+    double *sph = nullptr;
+    std::vector<double> sph_h;
     // medium
     float *kappa = nullptr, *albedo = nullptr, *g = nullptr;
     bool media_set = false;
@@ -240,6 +245,34 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
         p[e] = 0.0;
 }
 
+// Spherical-mesh streaming (photon_sph.h): persistent lanes or one thread per
+// packet; the source cell's J is summed in LDS when the launch gives it.
+template <bool PERSIST> __global__ void __launch_bounds__(256) k_sph(kp::kparams P)
+{
+    const bool lds = P.lds_n > 0;
+    if (lds) {
+        for (int k = threadIdx.x; k < P.lds_n; k += blockDim.x)
+            kp::kp_lds[k] = 0.0;
+        __syncthreads();
+    }
+    kp::f64::counters_t C;
+    if constexpr (PERSIST) {
+        kp::sph::persist(P, lds, C);
+    } else {
+        const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+        for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
+             slot += stride)
+            kp::sph::packet(P, slot, lds, C);
+    }
+    flush_counters(P, C);
+    if (lds) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < P.n_bands; b += blockDim.x)
+            if (P.J && kp::kp_lds[b] != 0.0)
+                kp::atomic_add_f64(kp::j_at(P, (uint64_t)P.src_gc, b), kp::kp_lds[b]);
+    }
+}
+
 template <class Pol, int WPS, bool ADJ>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
@@ -328,6 +361,7 @@ void free_all(kp_engine *e)
     dfree(e->h32);
     dfree(e->org32);
     dfree(e->vol32);
+    dfree(e->sph);
     dfree(e->kappa);
     dfree(e->albedo);
     dfree(e->g);
@@ -360,6 +394,43 @@ bool locate(const kp_engine *e, const double x[3], int &ib, int i[3])
             int c = (int)((x[a] - e->xf0_h[3 * b + a]) / e->dx0_h[3 * b + a]);
             i[a] = std::min(std::max(c, 0), e->n_cell - 1);
         }
+        return true;
+    }
+    return false;
+}
+
+// Cell of a Cartesian point on a spherical mesh, with the march's own face
+// tests (kp_sph.h): cell k of an axis when the point is at or beyond face k
+// and not at or beyond face k+1.
+bool sph_locate(const kp_engine *e, const double x[3], int &ib, int i[3])
+{
+    const int n = e->n_cell, m = n + 1;
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    auto beyond = [&](const double *T, int q, int k) {
+        if (q == 0)
+            return r2 >= T[k] * T[k];
+        if (q == 1)
+            return x[2] <= T[m + k] * std::sqrt(r2);
+        return x[1] * T[3 * m + k] - x[0] * T[2 * m + k] >= 0.0;
+    };
+    for (int b = 0; b < e->n_blocks; ++b) {
+        const double *T = e->sph_h.data() + (size_t)b * KP_SPH_TAB(n);
+        int c[3];
+        bool in = true;
+        for (int q = 0; q < 3 && in; ++q) {
+            c[q] = -1;
+            for (int k = 0; k < n; ++k)
+                if (beyond(T, q, k) && !beyond(T, q, k + 1)) {
+                    c[q] = k;
+                    break;
+                }
+            in = c[q] >= 0;
+        }
+        if (!in)
+            continue;
+        ib = b;
+        for (int q = 0; q < 3; ++q)
+            i[q] = c[q];
         return true;
     }
     return false;
@@ -461,6 +532,7 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
         P.h32 = e->h32;
         P.org32 = e->org32;
         P.vol32 = e->vol32;
+        P.sph = e->sph;
         P.n_cell = e->n_cell;
         P.n3 = e->n3;
         for (int a = 0; a < 3; ++a) {
@@ -482,12 +554,15 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
         P.total = total;
         P.packet_offset = run->packet_offset;
         int ib, ii[3];
-        if (!locate(e, src->x, ib, ii))
+        const bool sph_mesh = e->coord == KP_COORD_SPHERICAL;
+        if (!(sph_mesh ? sph_locate(e, src->x, ib, ii) : locate(e, src->x, ib, ii)))
             throw kp::error(KP_ERR_ARG, "source outside the mesh");
         for (int a = 0; a < 3; ++a) {
             P.src_x[a] = src->x[a];
             P.src_i[a] = ii[a];
-            P.src_xl[a] = (float)((src->x[a] - e->xf0_h[3 * ib + a]) / e->dx0_h[3 * ib + a]);
+            P.src_xl[a] = sph_mesh ? 0.0f
+                                   : (float)((src->x[a] - e->xf0_h[3 * ib + a]) /
+                                             e->dx0_h[3 * ib + a]);
         }
         P.src_ib = ib;
         P.lds_n = 0;
@@ -608,7 +683,16 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
     const uint64_t off0 = P.packet_offset;
     for (int k = 0; k < n_chunks; ++k) {
         P.packet_offset = off0 + (uint64_t)k * chunk_stride;
-        if (S.kind == KP_SCHED_HISTORY) {
+        if (P.sph) {
+            if (S.kind == KP_SCHED_HISTORY) {
+                hipLaunchKernelGGL(k_sph<false>, dim3((unsigned)pl.blocks), dim3(pl.T),
+                                   (size_t)P.lds_n * sizeof(double), stream, P);
+            } else {
+                hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.work);
+                hipLaunchKernelGGL(k_sph<true>, dim3((unsigned)pl.blocks), dim3(pl.T),
+                                   (size_t)P.lds_n * sizeof(double), stream, P);
+            }
+        } else if (S.kind == KP_SCHED_HISTORY) {
             if (run->precision == KP_F64)
                 hipLaunchKernelGGL(k_forward_f64, dim3((unsigned)pl.blocks), dim3(pl.T), 0,
                                    stream, P);
@@ -641,10 +725,22 @@ void check_forward_args(const kp_engine *e, const kp_source *src, const kp_run *
     const std::string w(who);
     if (!e || !src || !run || !out)
         throw kp::error(KP_ERR_ARG, w + ": null argument");
-    if (!e->media_set)
-        throw kp::error(KP_ERR_STATE, w + ": kp_set_media not called");
     if (run->n_bands < 1 || !run->bands)
         throw kp::error(KP_ERR_ARG, w + ": need at least one band");
+    if (e->coord == KP_COORD_SPHERICAL) {
+        // the reference's spherical photon streams without a medium
+        for (int b = 0; b < run->n_bands; ++b)
+            if (run->bands[b].kappa_scale != 0.0)
+                throw kp::error(KP_ERR_UNSUPPORTED,
+                                w + ": spherical meshes stream only (kappa_scale must be 0)");
+        if (run->n_incl != 0)
+            throw kp::error(KP_ERR_UNSUPPORTED, w + ": no peel-off images on spherical meshes");
+        if (run->precision != KP_F64)
+            throw kp::error(KP_ERR_UNSUPPORTED, w + ": spherical meshes run in FP64");
+        return;
+    }
+    if (!e->media_set)
+        throw kp::error(KP_ERR_STATE, w + ": kp_set_media not called");
     if (run->n_incl < 0 || run->n_incl > kp::kMaxIncl || (run->n_incl > 0 && !run->incl_deg))
         throw kp::error(KP_ERR_ARG, w + ": n_incl must be in [0,8]");
     if (run->n_incl > 0 && (e->periodic[0] || e->periodic[1] || e->periodic[2]))
@@ -708,6 +804,17 @@ kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **ou
             upload(e->nb_shift, m->nb_shift, 24 * nb);
             upload(e->d_lvl, m->d_lvl, 6 * nb);
             upload(e->phybnd, m->phybnd, nb);
+            if (m->coord == KP_COORD_SPHERICAL) {
+                if (!m->sph)
+                    throw kp::error(KP_ERR_ARG, "kp_engine_create: spherical mesh without table");
+                const size_t nt = (size_t)nb * KP_SPH_TAB(m->n_cell);
+                e->coord = KP_COORD_SPHERICAL;
+                e->sph_h.assign(m->sph, m->sph + nt);
+                e->sph = dmalloc<double>(nt);
+                upload(e->sph, m->sph, nt);
+            } else if (m->coord != KP_COORD_CARTESIAN) {
+                throw kp::error(KP_ERR_ARG, "kp_engine_create: unknown coordinate system");
+            }
             e->fb.work = dmalloc<unsigned long long>(1);
             KP_HIP(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device));
             std::vector<float> h(3 * nb), o(3 * nb), v(nb);
@@ -797,6 +904,8 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
     return kp::guard([&] {
         if (!e || !src || !run || !dL_dimage || !grads)
             throw kp::error(KP_ERR_ARG, "kp_adjoint: null argument");
+        if (e->coord == KP_COORD_SPHERICAL)
+            throw kp::error(KP_ERR_UNSUPPORTED, "kp_adjoint: no adjoint on spherical meshes");
         if (!e->media_set)
             throw kp::error(KP_ERR_STATE, "kp_adjoint: kp_set_media not called");
         if (run->precision != KP_F64 && run->precision != KP_F32)

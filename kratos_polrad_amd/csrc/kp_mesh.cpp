@@ -10,6 +10,8 @@
 //   physical-boundary flags ........... mesh/meshgen/meshgen.cpp:390-398
 //   SMR region -> block index ......... mesh/meshgen/meshgen.cpp:111-142,353-373
 //   face table (proxy setup) .......... modules/particle/types/proxy.h:22-62
+//   spherical axes / volumes .......... mesh/meshgen/spherical.h:9-140,
+//                                       mesh/meshgen/cylindrical.h:18-43
 // The tree is a std::map keyed in the same (level, x0, x1, x2) order, so the
 // ids handed out by kp_mesh_finalize are the reference's dist_mode=0 ids.
 
@@ -99,10 +101,13 @@ struct kp_mesh {
     std::array<double, 3> x_min{0, 0, 0}, x_max{1, 1, 1};
     std::map<reg_t, node_t, lex_less> tree;
     bool finalized = false;
+    int coord = KP_COORD_CARTESIAN;
+    double ratio = 1.0; // r width ratio of consecutive root-level cells
+    double dr_0 = -1.0; // first root-level r width (> 0: non-uniform r)
 
     // block table
     std::vector<reg_t> blocks;
-    std::vector<double> xf0, dx0, vol0;
+    std::vector<double> xf0, dx0, vol0, xf, sph;
     std::vector<int32_t> level, logic, nb_id;
     std::vector<uint8_t> nb_shift, phybnd;
     std::vector<int8_t> d_lvl;
@@ -206,6 +211,16 @@ struct kp_mesh {
         return out;
     }
 
+    // meshgen::base_t::location (meshgen.cpp:252-258); the non-uniform r axis
+    // of cyl_t::location (cylindrical.h:34-43): r_min + dr_0 (q^(n x) - 1)/(q - 1)
+    double location(double x_logic, int axis) const
+    {
+        if (axis != 0 || !(dr_0 > 0.0))
+            return x_min[axis] + x_logic * (x_max[axis] - x_min[axis]);
+        const double n_logic = x_logic * (double)(reg_max[0] * n_cell);
+        return x_min[0] + dr_0 / (ratio - 1.0) * (std::pow(ratio, n_logic) - 1.0);
+    }
+
     // meshgen.cpp:353-373
     int search_blk_idx(double loc, int lvl, int axis) const
     {
@@ -216,7 +231,7 @@ struct kp_mesh {
         const double i_ftot = (double)i_tot;
         while (i_r - i_l > 1) {
             const int i_m = (i_l + i_r) / 2;
-            const double xl = x_min[axis] + (i_m / i_ftot) * (x_max[axis] - x_min[axis]);
+            const double xl = location(i_m / i_ftot, axis);
             if (xl < loc)
                 i_l = i_m;
             else
@@ -246,9 +261,13 @@ struct kp_mesh {
             blocks.push_back(p.first);
         }
         const int nb = (int)blocks.size();
+        const int m = n_cell + 1;
+        const bool sph_mesh = coord == KP_COORD_SPHERICAL;
         xf0.assign(3 * nb, 0.0);
         dx0.assign(3 * nb, 0.0);
         vol0.assign(nb, 0.0);
+        xf.assign((size_t)3 * m * nb, 0.0);
+        sph.assign(sph_mesh ? (size_t)KP_SPH_TAB(n_cell) * nb : 0, 0.0);
         level.assign(nb, 0);
         logic.assign(3 * nb, 0);
         nb_id.assign(24 * nb, -1);
@@ -260,18 +279,50 @@ struct kp_mesh {
             level[b] = r.level;
             for (int a = 0; a < 3; ++a) {
                 logic[3 * b + a] = r.x[a];
-                // set_coord_axes (meshgen.cpp:286-306), uniform axes
+                // set_coord_axes (meshgen.cpp:286-306): uniform axes are
+                // xf0 + i*dx0, a non-uniform axis has dx0 = -1 and a table
                 auto loc = [&](double i) {
                     const double xl = (r.x[a] + i / n_cell) / (double)(reg_max[a] << r.level);
-                    return x_min[a] + xl * (x_max[a] - x_min[a]);
+                    return location(xl, a);
                 };
+                const bool uniform = !(a == 0 && dr_0 > 0.0);
                 xf0[3 * b + a] = loc(0);
-                dx0[3 * b + a] = loc(1) - xf0[3 * b + a];
+                dx0[3 * b + a] = uniform ? loc(1) - xf0[3 * b + a] : -1.0;
+                double *f = &xf[((size_t)3 * b + a) * m];
+                for (int k = 0; k <= n_cell; ++k)
+                    f[k] = uniform ? xf0[3 * b + a] + k * dx0[3 * b + a] : loc(k);
             }
-            double v = 1.0;
-            for (int a = 0; a < 3; ++a)
-                v *= dx0[3 * b + a];
-            vol0[b] = v;
+            if (sph_mesh) {
+                // sph_t::volume factors (spherical.h:31-42) and the face
+                // trigonometry of the spherical march; phi faces are reduced
+                // mod 2 pi so the periodic seam is one plane in every block
+                vol0[b] = -1.0;
+                const int n = n_cell;
+                double *t = &sph[(size_t)KP_SPH_TAB(n) * b];
+                const double *fr = &xf[((size_t)3 * b + 0) * m];
+                const double *ft = &xf[((size_t)3 * b + 1) * m];
+                const double *fp = &xf[((size_t)3 * b + 2) * m];
+                const double two_pi = 6.283185307179586476925286766559;
+                for (int k = 0; k <= n; ++k) {
+                    const double ph = std::fmod(fp[k], two_pi);
+                    // the equatorial face is the plane z = 0 exactly
+                    const double ct = std::cos(ft[k]);
+                    t[k] = fr[k];
+                    t[m + k] = std::fabs(ct) < 1e-15 ? 0.0 : ct;
+                    t[2 * m + k] = std::sin(ph);
+                    t[3 * m + k] = std::cos(ph);
+                }
+                for (int k = 0; k < n; ++k) {
+                    t[4 * m + k] = (std::pow(fr[k + 1], 3.0) - std::pow(fr[k], 3.0)) / 3.0;
+                    t[4 * m + n + k] = std::fabs(t[m + k] - t[m + k + 1]);
+                    t[4 * m + 2 * n + k] = fp[k + 1] - fp[k];
+                }
+            } else {
+                double v = 1.0;
+                for (int a = 0; a < 3; ++a)
+                    v *= dx0[3 * b + a];
+                vol0[b] = v;
+            }
             // associate_mesh phybnd (meshgen.cpp:390-398)
             const auto rl = region_max(r.level);
             for (int a = 0; a < 3; ++a)
@@ -340,6 +391,54 @@ kp_status kp_mesh_create(const int32_t root_blocks[3], int32_t n_cell, const dou
 }
 
 void kp_mesh_destroy(kp_mesh *mesh) { delete mesh; }
+
+kp_status kp_mesh_set_coord(kp_mesh *mesh, int32_t coord, double r_ratio)
+{
+    return kp::guard([&] {
+        if (!mesh)
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: null mesh");
+        if (mesh->finalized)
+            throw kp::error(KP_ERR_STATE, "kp_mesh_set_coord: mesh already finalized");
+        if (coord == KP_COORD_CARTESIAN) {
+            mesh->coord = coord;
+            mesh->ratio = 1.0;
+            mesh->dr_0 = -1.0;
+            return;
+        }
+        if (coord != KP_COORD_SPHERICAL)
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: unknown coordinate system");
+        const double pi = 3.14159265358979323846, two_pi = 2.0 * pi;
+        const auto &lo = mesh->x_min, &hi = mesh->x_max;
+        if (!(lo[0] > 0.0))
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: spherical meshes need r_min > 0");
+        if (!(lo[1] >= 0.0 && hi[1] <= pi + 1e-12))
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: theta limits must lie in [0, pi]");
+        const double wphi = hi[2] - lo[2];
+        if (!(wphi <= two_pi + 1e-12))
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: phi range wider than 2 pi");
+        if (!(wphi / (mesh->reg_max[2] * mesh->n_cell) < pi))
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: phi cells must be narrower than pi");
+        if (mesh->periodic[0] || mesh->periodic[1])
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: only phi may be periodic");
+        if (mesh->periodic[2] && std::fabs(wphi - two_pi) > 1e-9)
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: a periodic phi axis spans 2 pi");
+        if (!std::isfinite(r_ratio) || r_ratio == 0.0)
+            throw kp::error(KP_ERR_ARG, "kp_mesh_set_coord: bad r_ratio");
+        // cyl_t::read (cylindrical.h:18-32)
+        const int n_r = mesh->reg_max[0] * mesh->n_cell;
+        double q = r_ratio, dr = -1.0;
+        if (std::fabs(q - 1.0) >= 1e-6) {
+            if (q < 0.0)
+                q = std::pow(hi[0] / lo[0], 1.0 / n_r);
+            dr = (hi[0] - lo[0]) / ((std::pow(q, n_r) - 1.0) / (q - 1.0));
+        } else {
+            q = 1.0;
+        }
+        mesh->coord = coord;
+        mesh->ratio = q;
+        mesh->dr_0 = dr;
+    });
+}
 
 kp_status kp_mesh_refine(kp_mesh *mesh, int32_t level, const int32_t *idx, int64_t n)
 {
@@ -424,6 +523,9 @@ kp_status kp_mesh_describe(const kp_mesh *mesh, kp_mesh_desc *d)
         d->nb_shift = mesh->nb_shift.data();
         d->d_lvl = mesh->d_lvl.data();
         d->phybnd = mesh->phybnd.data();
+        d->coord = mesh->coord;
+        d->xf = mesh->xf.data();
+        d->sph = mesh->coord == KP_COORD_SPHERICAL ? mesh->sph.data() : nullptr;
     });
 }
 
@@ -437,6 +539,29 @@ kp_status kp_mesh_locate(const kp_mesh *mesh, const double x[3], int32_t *ib, in
         *ib = -1;
         i[0] = i[1] = i[2] = 0;
         const int n = mesh->n_cell;
+        if (mesh->coord != KP_COORD_CARTESIAN) {
+            // geo_t::idx over the face tables: the cell k with xf[k] <= x < xf[k+1]
+            const int m = n + 1;
+            for (size_t b = 0; b < mesh->blocks.size(); ++b) {
+                int c[3];
+                bool in = true;
+                for (int a = 0; a < 3 && in; ++a) {
+                    const double *f = &mesh->xf[(3 * b + a) * m];
+                    in = (x[a] >= f[0] && x[a] < f[n]);
+                    int k = 0;
+                    while (in && k + 1 < n && f[k + 1] <= x[a])
+                        ++k;
+                    c[a] = k;
+                }
+                if (!in)
+                    continue;
+                *ib = (int32_t)b;
+                for (int a = 0; a < 3; ++a)
+                    i[a] = c[a];
+                return;
+            }
+            return;
+        }
         for (size_t b = 0; b < mesh->blocks.size(); ++b) {
             bool in = true;
             for (int a = 0; a < 3 && in; ++a) {

@@ -20,7 +20,8 @@ class Mesh:
     """
 
     def __init__(self, root_blocks=(4, 4, 4), n_cell=16, x_min=(0.0, 0.0, 0.0),
-                 x_max=(1.0, 1.0, 1.0), periodic=(0, 0, 0)):
+                 x_max=(1.0, 1.0, 1.0), periodic=(0, 0, 0), coord="cartesian",
+                 r_ratio=1.0):
         L = _abi.lib()
         self._h = C.c_void_p()
         rb = (C.c_int32 * 3)(*root_blocks)
@@ -34,6 +35,11 @@ class Mesh:
         self.x_max = np.array(x_max, dtype=np.float64)
         self.periodic = tuple(int(p) for p in periodic)
         self.desc = None
+        if coord not in ("cartesian", "spherical"):
+            raise ValueError("coord must be 'cartesian' or 'spherical'")
+        self.coord = coord
+        if coord == "spherical":
+            _abi.check(L.kp_mesh_set_coord(self._h, _abi.KP_COORD_SPHERICAL, float(r_ratio)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -77,6 +83,10 @@ class Mesh:
         self.nb_shift = view(d.nb_shift, 24 * nb, np.uint8).reshape(nb, 24)
         self.d_lvl = view(d.d_lvl, 6 * nb, np.int8).reshape(nb, 6)
         self.phybnd = view(d.phybnd, nb, np.uint8)
+        m = self.n_cell + 1
+        self.xf = view(d.xf, 3 * m * nb, np.float64).reshape(nb, 3, m)
+        if self.coord == "spherical":
+            self.sph = view(d.sph, _abi.sph_tab_len(self.n_cell) * nb, np.float64).reshape(nb, -1)
         return self
 
     @property
@@ -94,8 +104,16 @@ class Mesh:
         return ib.value, tuple(i)
 
     def cell_centers(self) -> np.ndarray:
-        """[n_cells, 3] cell-centre coordinates in block-table cell order."""
+        """[n_cells, 3] cell-centre coordinates in block-table cell order
+        (mesh coordinates: face midpoints per axis, i.e. (r, theta, phi) on a
+        spherical mesh)."""
         n = self.n_cell
+        if self.coord == "spherical":
+            mid = 0.5 * (self.xf[:, :, :-1] + self.xf[:, :, 1:])  # [nb, 3, n]
+            c0 = np.broadcast_to(mid[:, 0][:, None, None, :], (self.n_blocks, n, n, n))
+            c1 = np.broadcast_to(mid[:, 1][:, None, :, None], (self.n_blocks, n, n, n))
+            c2 = np.broadcast_to(mid[:, 2][:, :, None, None], (self.n_blocks, n, n, n))
+            return np.stack([c0, c1, c2], axis=-1).reshape(-1, 3)
         i = np.arange(n, dtype=np.float64) + 0.5
         # cell (i0,i1,i2) -> linear i0 + n*(i1 + n*i2)
         g2, g1, g0 = np.meshgrid(i, i, i, indexing="ij")
@@ -103,4 +121,12 @@ class Mesh:
         return (self.xf0[:, None, :] + loc[None, :, :] * self.dx0[:, None, :]).reshape(-1, 3)
 
     def cell_volumes(self) -> np.ndarray:
+        if self.coord == "spherical":
+            n, m = self.n_cell, self.n_cell + 1
+            fr = self.sph[:, 4 * m:4 * m + n]
+            ft = self.sph[:, 4 * m + n:4 * m + 2 * n]
+            fp = self.sph[:, 4 * m + 2 * n:4 * m + 3 * n]
+            # cell (i0,i1,i2) -> i0 + n*(i1 + n*i2); volume (f_r*f_th)*f_phi
+            v = (fr[:, None, None, :] * ft[:, None, :, None]) * fp[:, :, None, None]
+            return v.reshape(-1)
         return np.repeat(self.vol0, self.n3)

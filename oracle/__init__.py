@@ -7,8 +7,9 @@ this package, as the checker or the timed CPU baseline.  The product
 The C sources restate the reference march of
 src/modules/particle/radiation/photon.h:20-125, types/proxy.h:64-121,
 types/particle_base.h:37-144 and mesh/geometry/geometry.h:68-83, plus the
-north-star physics (see kp_oracle.h for the list and the two documented
-deviations).
+north-star physics (see kp_oracle.h for the list and the documented
+deviations), and the spherical march of radiation/photon_sph.h:10-266
+(kp_sph_oracle.c).
 """
 from __future__ import annotations
 
@@ -72,6 +73,13 @@ def lib():
         L.or_bin_by_key.argtypes = [P, C.c_int64, C.c_int32, P, P, P]
         L.or_gather.restype = None
         L.or_gather.argtypes = [P, C.c_int64, P, C.c_int64, P]
+        L.or_sph_forward.restype = C.c_int
+        L.or_sph_forward.argtypes = [C.POINTER(_abi.MeshDesc), C.POINTER(_abi.Source),
+                                     C.POINTER(_abi.Run), P, P, P, C.c_int]
+        L.or_sph_trace.restype = C.c_int
+        L.or_sph_trace.argtypes = L.or_trace.argtypes
+        L.or_sph_locate.restype = C.c_int
+        L.or_sph_locate.argtypes = L.or_locate.argtypes
         _lib = L
     return _lib
 
@@ -122,6 +130,52 @@ def trace(mesh, x, d, max_steps=100000):
     k = lib().or_trace(C.byref(mesh.desc), (C.c_double * 3)(*x), (C.c_double * 3)(*d),
                        max_steps, _ptr(ib), _ptr(cell), _ptr(ln), xe, C.byref(st))
     return ib[:k], cell[:k], ln[:k], np.array(xe[:]), st.value
+
+
+def sph_forward(mesh, source_x, run_spec, weight=1.0, J=True, end_state=False,
+                n_threads=None):
+    """Spherical-mesh streaming run (or_sph_forward) on the same inputs as
+    Engine.forward on a spherical mesh; returns numpy outputs."""
+    from kratos_polrad_amd import _abi
+    from kratos_polrad_amd.engine import source_c
+
+    r, keep = run_spec.to_c()
+    nb = len(run_spec.bands)
+    out = {"counters": np.zeros(_abi.KP_NCOUNTERS, dtype=np.uint64)}
+    if J:
+        out["J"] = np.zeros(mesh.n_cells * nb, dtype=np.float64)
+    if end_state:
+        out["end_state"] = np.zeros((run_spec.total_packets, _abi.KP_ES_STRIDE))
+    src = source_c(source_x, weight)
+    if n_threads is None:
+        n_threads = os.cpu_count() or 1
+    rc = lib().or_sph_forward(C.byref(mesh.desc), C.byref(src), C.byref(r), _ptr(out.get("J")),
+                              _ptr(out["counters"]), _ptr(out.get("end_state")), int(n_threads))
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"or_sph_forward failed: {rc}")
+    return out
+
+
+def sph_trace(mesh, x, d, max_steps=100000):
+    """Spherical march of one ray from Cartesian x along d:
+    (ib[], cell[], length[], x_end, status)."""
+    ib = np.zeros(max_steps, dtype=np.int32)
+    cell = np.zeros(max_steps, dtype=np.int32)
+    ln = np.zeros(max_steps, dtype=np.float64)
+    xe = (C.c_double * 3)()
+    st = C.c_int()
+    k = lib().or_sph_trace(C.byref(mesh.desc), (C.c_double * 3)(*x), (C.c_double * 3)(*d),
+                           max_steps, _ptr(ib), _ptr(cell), _ptr(ln), xe, C.byref(st))
+    return ib[:k], cell[:k], ln[:k], np.array(xe[:]), st.value
+
+
+def sph_locate(mesh, x):
+    """Cell of Cartesian point x on a spherical mesh: (ib, (i0, i1, i2)), ib=-1 outside."""
+    ib = C.c_int32()
+    i = (C.c_int32 * 3)()
+    lib().or_sph_locate(C.byref(mesh.desc), (C.c_double * 3)(*x), C.byref(ib), i)
+    return ib.value, tuple(i)
 
 
 def philox(ctr, k0, k1):

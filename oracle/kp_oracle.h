@@ -63,6 +63,17 @@ int or_trace(const kp_mesh_desc *mesh, const double x[3], const double d[3],
              int max_steps, int32_t *out_ib, int32_t *out_cell, double *out_len,
              double x_end[3], int *status);
 
+/* Spherical meshes (kp_sph_oracle.c): the reference's spherical photon march
+ * (photon_sph.h:10-266) as pure streaming with the path-length tally.  The
+ * source point and traced rays are Cartesian; or_sph_locate finds the cell
+ * containing a Cartesian point with the march's own face comparisons. */
+int or_sph_forward(const kp_mesh_desc *mesh, const kp_source *src, const kp_run *run,
+                   double *J, uint64_t *counters, double *end_state, int n_threads);
+int or_sph_trace(const kp_mesh_desc *mesh, const double x[3], const double d[3],
+                 int max_steps, int32_t *out_ib, int32_t *out_cell, double *out_len,
+                 double x_end[3], int *status);
+int or_sph_locate(const kp_mesh_desc *mesh, const double x[3], int32_t *ib, int32_t i[3]);
+
 /* Source cell by brute-force point location (geo_t::idx, geometry.h:120-140). */
 int or_locate(const kp_mesh_desc *mesh, const double x[3], int32_t *ib, int32_t i[3]);
 
