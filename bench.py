@@ -49,6 +49,9 @@ def parse():
                    help="cells of the per-cell finite-difference check of the adjoint (0: off)")
     p.add_argument("--batched-scale", type=float, default=1.0,
                    help="scale of the per-band packet counts of the 64-band workload")
+    p.add_argument("--skip-sph", action="store_true")
+    p.add_argument("--sph-packets", type=float, default=1e8,
+                   help="packets per step of the spherical-mesh streaming workload")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 512 MiB device-to-device copy (3 times) so "
                         "rocprofv3 counters can be scaled by a kernel of known traffic")
@@ -446,6 +449,78 @@ def compaction(n=100_000_000, unit=64, n_bins=2234, p_keep=0.7):
     return res
 
 
+def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
+    """The reference's spherical photon (photon_sph.h) on the spherical shell
+    workload: pure streaming with the path-length tally, FP64, bit-exact with
+    the CPU restatement.  A step is one kp_forward launch over `packets`
+    packets; value = packet-path crossings / wall time."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi, configs
+    from kratos_polrad_amd.engine import Engine, RunSpec
+
+    n = int(packets)
+    w = configs.config_sph(n_packets=n)
+    eng = Engine(w.mesh)
+    stream = torch.cuda.current_stream()
+    res = {"workload": w.name, "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
+           "levels": int(w.mesh.level.max()) + 1, "packets_per_step": n}
+    t0 = time.perf_counter()
+    res["schedule"] = eng.autotune(w.source, w.run, sample_packets=10_000_000)
+    res["autotune_seconds"] = time.perf_counter() - t0
+    run = RunSpec(**w.run.__dict__)
+    out = eng.alloc_outputs(run)
+    for k in range(warmup):
+        run.packet_offset = (1 << 50) + k * n
+        eng.forward(w.source, run, out)
+    torch.cuda.synchronize()
+    for v in out.values():
+        v.zero_()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for k in range(steps):
+        run.packet_offset = k * n
+        eng.forward(w.source, run, out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = ev0.elapsed_time(ev1) / 1e3
+    c = out["counters"].cpu().numpy().astype(np.float64)
+    res["value"] = float(c[_abi.KP_C_CROSS_MAIN] / wall)
+    res["unit"] = "cell-crossings/s"
+    res["ms_per_step"] = 1e3 * wall / steps
+    res["kernel_ms_per_step"] = 1e3 * kern / steps
+    res["crossings_per_packet"] = float(c[_abi.KP_C_CROSS_MAIN] / c[_abi.KP_C_PACKETS])
+    res["escaped_fraction"] = float(c[_abi.KP_C_ESCAPED] / c[_abi.KP_C_PACKETS])
+    # J-atomic traffic: one FP64 read-modify-write (16 B) per crossing
+    res["j_atomic_GBps"] = float(16.0 * c[_abi.KP_C_CROSS_MAIN] / kern / 1e9)
+    if not no_cpu:
+        import oracle
+
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        npk, dt = 20000, 0.0
+        while True:
+            r = RunSpec(**{**w.run.__dict__, "bands": [type(w.run.bands[0])(
+                **{**w.run.bands[0].__dict__, "n_packets": npk})]})
+            t0 = time.perf_counter()
+            o = oracle.sph_forward(w.mesh, w.source, r, J=True, n_threads=cores)
+            dt = time.perf_counter() - t0
+            if dt >= 0.5 * cpu_seconds or npk >= 5e7:
+                break
+            npk = int(min(5e7, npk * max(2.0, cpu_seconds / max(dt, 1e-3))))
+        res["cpu_baseline"] = {
+            "value": float(o["counters"][_abi.KP_C_CROSS_MAIN] / dt), "unit": "cell-crossings/s",
+            "cores": cores, "kind": "port",
+            "sample": f"{npk} packets of the same workload (FP64 spherical oracle, "
+                      f"pthreads), {dt:.1f} s"}
+    del out, eng
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -564,6 +639,10 @@ def main():
 
     if not args.skip_large:
         res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000))
+
+    if not args.skip_sph:
+        res["spherical"] = spherical(args.steps, args.warmup, args.sph_packets,
+                                     args.cpu_seconds / 2, args.no_cpu)
 
     if not args.no_cpu:
         import oracle

@@ -252,6 +252,18 @@ typedef struct kp_schedule {
      * compiler must fit (0 = default: FP32 4, FP64 unconstrained; 1 =
      * unconstrained; FP32: 4,5,6; FP64: 3,4) */
     int32_t waves_per_simd;
+    /* 1 = march the packets of a launch in emission-direction order: each
+     * packet's key is its band and its emission direction (up to 2^30
+     * equal-solid-angle cells of the (mu, phi) square in Morton order, from
+     * the emission draws), radix-sorted before the transport kernel, so the
+     * lanes of a wave fly neighbouring rays: their steps stay in lockstep and
+     * equal-cell tallies of neighbouring lanes are summed before one atomic.
+     * Histories are unchanged (end_state stays indexed by slot); tallies
+     * differ by summation order only.  Takes about 26 B of scratch per
+     * packet; launches of 2^31 packets or more run in slot order.  0 = slot
+     * order.  Engines on spherical meshes start with the one-thread-per-packet
+     * schedule in this order. */
+    int32_t coherent;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

@@ -105,6 +105,7 @@ class Schedule(C.Structure):
         ("img_replicas", C.c_int32),
         ("event_batch", C.c_int32),
         ("waves_per_simd", C.c_int32),
+        ("coherent", C.c_int32),
     ]
 
 

@@ -1,0 +1,18 @@
+// kp_compact.h -- device sorting helpers shared inside the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace kp {
+
+// Stable LSD radix sort (8-bit digits) of n non-negative keys below 2^bits
+// (bits in [1, 30]), giving the permutation: order[p] = index of the p-th key
+// in ascending order.  Kernels only, on `stream` (capturable into a graph);
+// ws needs kp_compact_workspace_bytes(n, 1) bytes; n < 2^31.
+void sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void *ws,
+                size_t ws_bytes, hipStream_t stream);
+
+} // namespace kp

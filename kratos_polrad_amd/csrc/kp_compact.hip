@@ -17,6 +17,7 @@
 #include <string>
 #include <type_traits>
 
+#include "kp_compact.h"
 #include "kp_error.h"
 
 #define KP_HIPC(call)                                                                       \
@@ -432,7 +433,59 @@ template <class F> void by_word(int32_t unit, const void *a, const void *b, F &&
         f((uint32_t *)nullptr, unit / 4);
 }
 
+__global__ void __launch_bounds__(kThreads) k_copy_order(const uint32_t *sidx, uint64_t n,
+                                                       int32_t *order)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        order[p] = (int32_t)sidx[p];
+}
+
 } // namespace
+
+void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void *ws,
+                    size_t ws_bytes, hipStream_t s)
+{
+    if (n >= (1ull << 31) || bits < 1 || bits > 30)
+        throw kp::error(KP_ERR_ARG, "sort_order: n must be below 2^31 and bits in [1,30]");
+    if (n == 0)
+        return;
+    const bin_layout L = bin_ws(n);
+    if (!keys || !order || !ws || ws_bytes < L.total)
+        throw kp::error(KP_ERR_ARG, "sort_order: bad buffers");
+    char *w = (char *)ws;
+    uint32_t *ka = (uint32_t *)(w + L.ka), *kb = (uint32_t *)(w + L.kb);
+    uint32_t *ia = (uint32_t *)(w + L.ia), *ib = (uint32_t *)(w + L.ib);
+    uint32_t *hist = (uint32_t *)(w + L.hist);
+    const uint64_t nt = (n + kTile - 1) / kTile;
+    scan_ws sw{(uint32_t *)(w + L.sums), scan_chunks(256 * nt)};
+    const int32_t lim = (int32_t)(1u << 30); // every key passes bin_key unchanged
+    const uint32_t *kin = nullptr, *iin = nullptr;
+    uint32_t *kout = ka, *iout = ia;
+    for (int p = 0; p < (bits + 7) / 8; ++p) {
+        const int shift = 8 * p;
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nt), dim3(kThreads), 0, s, keys,
+                               (const uint32_t *)nullptr, n, lim, shift, hist, nt);
+        else
+            hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, kin, n, lim, shift, hist, nt);
+        exscan(hist, 256 * nt, sw, nullptr, s);
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, (const uint32_t *)nullptr, (const uint32_t *)nullptr, kout,
+                               iout, n, lim, shift, hist, nt);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, kin, iin, kout, iout, n, lim, shift, hist, nt);
+        kin = kout;
+        iin = iout;
+        kout = (kout == ka) ? kb : ka;
+        iout = (iout == ia) ? ib : ia;
+    }
+    hipLaunchKernelGGL(k_copy_order, dim3(grid_for(n)), dim3(kThreads), 0, s, iin, n, order);
+    KP_HIPC(hipGetLastError());
+}
 
 extern "C" {
 

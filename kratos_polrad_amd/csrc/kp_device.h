@@ -78,6 +78,7 @@ struct kparams {
     double *end_state;
     // persistent schedule
     unsigned long long *work; // slot counter, zeroed before the launch
+    const int32_t *order;     // claim index -> slot (emission-direction order), or null
     uint32_t chunk;           // slots a wave claims per refill
     uint32_t batch;           // waiting lanes that trigger an action phase
     // J and image are replicated R times; workgroup b accumulates into
@@ -120,6 +121,12 @@ __device__ __forceinline__ double *img_base(const kparams &P)
 __device__ __forceinline__ void atomic_add_f64(double *p, double v)
 {
     unsafeAtomicAdd(p, v); // native global_atomic_add_f64 (coarse-grained memory)
+}
+
+// Slot of the k-th claimed packet of a launch.
+__device__ __forceinline__ uint64_t slot_at(const kparams &P, uint64_t k)
+{
+    return P.order ? (uint64_t)(uint32_t)P.order[k] : k;
 }
 
 // Band of a global packet slot: the last b with band_start[b] <= slot

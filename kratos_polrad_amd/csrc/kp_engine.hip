@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "kp_compact.h"
 #include "kp_device.h"
 #include "kp_error.h"
 #include "kp_adjoint.h"
@@ -38,6 +39,10 @@ struct fwd_bufs {
     unsigned long long *work = nullptr;
     double *j_rep = nullptr, *img_rep = nullptr;
     size_t j_cap = 0, img_cap = 0; // elements
+    // emission-direction order: keys, permutation, sort workspace
+    int32_t *keys = nullptr, *order = nullptr;
+    void *ws = nullptr;
+    size_t ord_cap = 0, ws_cap = 0; // elements / bytes
 };
 
 struct kp_engine {
@@ -66,7 +71,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0};  // last: waves_per_simd
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0};
     int n_cu = 0;
     fwd_bufs fb;                        // forward launch scratch
     // adjoint: per-thread rows of peel-off contributions, gradient replicas
@@ -160,8 +165,9 @@ __global__ void __launch_bounds__(256) k_forward_f64(kp::kparams P)
 {
     kp::f64::counters_t C;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
-         slot += stride) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.total;
+         k += stride) {
+        const uint64_t slot = kp::slot_at(P, k);
         const int band = kp::band_of(P, slot);
         const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
         kp::f64::packet(P, band, pid, slot, C);
@@ -173,8 +179,9 @@ __global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
 {
     kp::f32::counters_t C;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
-         slot += stride) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.total;
+         k += stride) {
+        const uint64_t slot = kp::slot_at(P, k);
         const int band = kp::band_of(P, slot);
         const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
         kp::f32::packet(P, band, pid, slot, C);
@@ -245,9 +252,41 @@ __global__ void __launch_bounds__(256) k_fill0(double *p, uint64_t n)
         p[e] = 0.0;
 }
 
+// Emission-direction sort key of every slot of a launch: the band, then the
+// top `bits` bits of the packet's event-0 Philox words 0 (mu = 2u-1) and 1
+// (azimuth) -- equal-solid-angle cells of the (mu, phi) square -- interleaved
+// in Morton order, so that neighbouring keys are neighbouring directions.
+__global__ void __launch_bounds__(256) k_dir_keys(kp::kparams P, int bits, int32_t *keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < P.total; s += stride) {
+        const int band = kp::band_of(P, s);
+        const uint64_t pid = P.packet_offset + (s - P.band_start[band]);
+        uint32_t rn[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), 0u, 0u};
+        kp_philox4x32_10(rn, P.seed, (uint32_t)band);
+        const uint32_t mu = rn[0] >> (32 - bits), ph = rn[1] >> (32 - bits);
+        uint32_t m = 0;
+        for (int b = 0; b < bits; ++b)
+            m |= (((ph >> b) & 1u) << (2 * b)) | (((mu >> b) & 1u) << (2 * b + 1));
+        keys[s] = (int32_t)(((uint32_t)band << (2 * bits)) | m);
+    }
+}
+
+// Direction bits per axis of the sort key: as many as fit 30 bits beside the
+// band (15 for one band: ~2^30 cells, finer than the packets of a launch).
+int dir_key_bits(int n_bands, int &total_bits)
+{
+    int band_bits = 0;
+    while ((1 << band_bits) < n_bands)
+        ++band_bits;
+    const int b = std::min(15, (30 - band_bits) / 2);
+    total_bits = band_bits + 2 * b;
+    return b;
+}
+
 // Spherical-mesh streaming (photon_sph.h): persistent lanes or one thread per
 // packet; the source cell's J is summed in LDS when the launch gives it.
-template <bool PERSIST> __global__ void __launch_bounds__(256) k_sph(kp::kparams P)
+template <bool PERSIST> __global__ void __launch_bounds__(256, 4) k_sph(kp::kparams P)
 {
     const bool lds = P.lds_n > 0;
     if (lds) {
@@ -256,14 +295,10 @@ template <bool PERSIST> __global__ void __launch_bounds__(256) k_sph(kp::kparams
         __syncthreads();
     }
     kp::f64::counters_t C;
-    if constexpr (PERSIST) {
+    if constexpr (PERSIST)
         kp::sph::persist(P, lds, C);
-    } else {
-        const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-        for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < P.total;
-             slot += stride)
-            kp::sph::packet(P, slot, lds, C);
-    }
+    else
+        kp::sph::history(P, lds, C);
     flush_counters(P, C);
     if (lds) {
         __syncthreads();
@@ -347,6 +382,10 @@ void free_bufs(fwd_bufs &b)
     dfree(b.j_rep);
     dfree(b.img_rep);
     b.j_cap = b.img_cap = 0;
+    dfree(b.keys);
+    dfree(b.order);
+    dfree(b.ws);
+    b.ord_cap = b.ws_cap = 0;
 }
 
 void free_all(kp_engine *e)
@@ -603,6 +642,8 @@ struct fwd_plan {
     int T = 256;
     uint64_t nJ = 0, nI = 0;
     bool j_band_major = false; // J accumulated band-major in replicas, folded by k_fold_t
+    bool coherent = false;     // packets marched in emission-direction order
+    int dir_bits = 0, key_bits = 0;
 };
 
 fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total,
@@ -658,6 +699,29 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         if (n * sizeof(double) <= kMaxLdsAcc)
             P.lds_n = (int)n;
     }
+    // emission-direction order: key and permutation arrays and the sort
+    // workspace, grown to this launch
+    pl.coherent = S.coherent != 0 && total < (1ull << 31); // else slot order
+    if (pl.coherent) {
+        pl.dir_bits = dir_key_bits(run->n_bands, pl.key_bits);
+        if (pl.dir_bits < 4)
+            throw kp::error(KP_ERR_UNSUPPORTED, "coherent order supports up to 2^22 bands");
+        if (fb.ord_cap < total) {
+            dfree(fb.keys);
+            dfree(fb.order);
+            fb.ord_cap = 0;
+            fb.keys = dmalloc<int32_t>(total);
+            fb.order = dmalloc<int32_t>(total);
+            fb.ord_cap = total;
+        }
+        const size_t wsb = kp_compact_workspace_bytes((int64_t)total, 1);
+        if (fb.ws_cap < wsb) {
+            dfree(fb.ws);
+            fb.ws_cap = 0;
+            KP_HIP(hipMalloc(&fb.ws, wsb));
+            fb.ws_cap = wsb;
+        }
+    }
     if (!fb.work) {
         fb.work = dmalloc<unsigned long long>(1);
         KP_HIP(hipMemsetAsync(fb.work, 0, sizeof(unsigned long long), stream));
@@ -683,6 +747,12 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
     const uint64_t off0 = P.packet_offset;
     for (int k = 0; k < n_chunks; ++k) {
         P.packet_offset = off0 + (uint64_t)k * chunk_stride;
+        if (pl.coherent) {
+            hipLaunchKernelGGL(k_dir_keys, dim3(2048), dim3(256), 0, stream, P, pl.dir_bits,
+                               fb.keys);
+            kp::sort_order(fb.keys, P.total, pl.key_bits, fb.order, fb.ws, fb.ws_cap, stream);
+            P.order = fb.order;
+        }
         if (P.sph) {
             if (S.kind == KP_SCHED_HISTORY) {
                 hipLaunchKernelGGL(k_sph<false>, dim3((unsigned)pl.blocks), dim3(pl.T),
@@ -809,6 +879,9 @@ kp_status kp_engine_create(const kp_mesh_desc *m, int32_t device, kp_engine **ou
                     throw kp::error(KP_ERR_ARG, "kp_engine_create: spherical mesh without table");
                 const size_t nt = (size_t)nb * KP_SPH_TAB(m->n_cell);
                 e->coord = KP_COORD_SPHERICAL;
+                // streaming histories run best in lockstep, in emission-direction order
+                e->sched.kind = KP_SCHED_HISTORY;
+                e->sched.coherent = 1;
                 e->sph_h.assign(m->sph, m->sph + nt);
                 e->sph = dmalloc<double>(nt);
                 upload(e->sph, m->sph, nt);
@@ -1181,10 +1254,33 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                     }
                 }
             };
+            const bool sph_mesh = e->coord == KP_COORD_SPHERICAL;
+            // schedule kind and packet order together: lockstep histories pay
+            // off in emission-direction order, which persistent lanes lose.
+            // With observers the one-thread-per-packet kernels march every
+            // peel-off ray inside the history, unbatched: not a candidate.
+            for (int kind : {KP_SCHED_PERSISTENT, KP_SCHED_HISTORY})
+                for (int coh : {0, 1}) {
+                    if ((cur.kind == kind && cur.coherent == coh) ||
+                        (coh && total >= (1ull << 31)) ||
+                        (kind == KP_SCHED_HISTORY && run->n_incl > 0))
+                        continue;
+                    kp_schedule c = cur;
+                    c.kind = kind;
+                    c.coherent = coh;
+                    const float ms = time_it(c);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        cur = c;
+                    }
+                }
             try_knob(&kp_schedule::j_replicas, {0, 1, 2, 3, 4, 8});
-            try_knob(&kp_schedule::img_replicas, {0, 16, 64});
+            if (!sph_mesh) // the spherical kernel has no images and one register budget
+                try_knob(&kp_schedule::img_replicas, {0, 16, 64});
             try_knob(&kp_schedule::event_batch, {4, 8, 16});
-            if (run->precision == KP_F64)
+            if (sph_mesh)
+                ;
+            else if (run->precision == KP_F64)
                 try_knob(&kp_schedule::waves_per_simd, {0, 3, 4});
             else
                 try_knob(&kp_schedule::waves_per_simd, {0, 1, 5});
@@ -1223,6 +1319,8 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
         if (s->j_replicas < 0 || s->j_replicas > 256 || s->img_replicas < 0 ||
             s->img_replicas > 256)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: replicas must be in [0,256]");
+        if (s->coherent != 0 && s->coherent != 1)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: coherent must be 0 or 1");
         e->sched = *s;
     });
 }

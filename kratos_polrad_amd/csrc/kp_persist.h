@@ -670,7 +670,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 }
                 if (L.mode == M_NEED) {
                     if (my < P.total) {
-                        L.slot = my;
+                        L.slot = slot_at(P, my);
                         emit_packet<Pol, ADJ>(P, L);
                     } else {
                         L.mode = M_DONE;

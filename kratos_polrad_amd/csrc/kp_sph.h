@@ -31,9 +31,9 @@ struct ray_t {
 };
 
 // the current cell's faces: r, cos(theta), sin/cos(phi) at the lower and
-// upper face of each axis
+// upper face of each axis, and its volume
 struct faces_t {
-    double rl, rr, cl, cr, sl, sr, ql, qr;
+    double rl, rr, cl, cr, sl, sr, ql, qr, vol;
 };
 
 __device__ __forceinline__ const double *tab(const kparams &P, int ib)
@@ -54,6 +54,8 @@ __device__ __forceinline__ void load_faces(const kparams &P, const ray_t &r, fac
     f.sr = T[2 * m + r.i2 + 1];
     f.ql = T[3 * m + r.i2];
     f.qr = T[3 * m + r.i2 + 1];
+    const int n = P.n_cell;
+    f.vol = (T[4 * m + r.i0] * T[4 * m + n + r.i1]) * T[4 * m + 2 * n + r.i2];
 }
 
 // sph_t::volume (spherical.h:31-42) from the per-axis factors
@@ -313,11 +315,14 @@ __device__ __forceinline__ void start(const kparams &P, ray_t &r, int band, uint
     r.i2 = P.src_i[2];
 }
 
-// One march step with its tally.  Returns 0 while the packet goes on, else
-// its KP_END_* status.
-__device__ __forceinline__ int step(const kparams &P, ray_t &r, int band, int &steps,
-                                    uint64_t &ncross, bool lds)
+// One march step.  Returns 0 while the packet goes on, else its KP_END_*
+// status; the step's tally (cell gc, value v) is handed back for deposit()
+// (gc = -1: none).
+__device__ __forceinline__ int step(const kparams &P, ray_t &r, int &steps, uint64_t &ncross,
+                                    int64_t &gc, double &v)
 {
+    gc = -1;
+    v = 0.0;
     if (steps <= 0)
         return KP_END_STEPS;
     faces_t f;
@@ -327,10 +332,8 @@ __device__ __forceinline__ int step(const kparams &P, ray_t &r, int band, int &s
     const int ap = get_dl(r, f, l, s0, s1, s2);
     if (ap < 0)
         return KP_END_LOST;
-    if (P.J) {
-        const int64_t gc = (int64_t)r.ib * P.n3 + cell_index(P, r);
-        j_deposit(P, lds, gc, band, P.w0 * (l / volume(P, r)));
-    }
+    gc = (int64_t)r.ib * P.n3 + cell_index(P, r);
+    v = P.w0 * (l / f.vol);
     ncross += 1;
     steps -= 1;
     const int st = cross(P, r, f, ap, l, s0, s1, s2);
@@ -339,6 +342,48 @@ __device__ __forceinline__ int step(const kparams &P, ray_t &r, int band, int &s
     if (st == ST_LOST)
         return KP_END_LOST;
     return 0;
+}
+
+// J deposit of every lane of the wave (called by all 64 lanes together;
+// `has` marks lanes with a tally).  The source cell goes to the workgroup's
+// LDS accumulator.  With `runs`, lanes whose J element equals their lower
+// neighbour's are summed first by a segmented scan across the wave and the
+// last lane of each run issues one atomic: in emission-direction order the
+// lanes of a wave fly neighbouring rays, and FP64 atomics on one address
+// serialise in the L2.
+__device__ __forceinline__ void deposit(const kparams &P, bool lds, bool runs, bool has,
+                                        int64_t gc, int band, double v)
+{
+    if (!P.J)
+        return;
+    if (has && lds && gc == P.src_gc) {
+        atomicAdd(&kp_lds[band], v);
+        has = false;
+    }
+    if (!runs) {
+        if (has)
+            atomic_add_f64(j_at(P, (uint64_t)gc, band), v);
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    // J element as the run key; lanes without a tally get keys of their own
+    const uint64_t key = has ? (uint64_t)gc * (uint32_t)P.j_cs + (uint64_t)band * (uint32_t)P.j_bs
+                             : ~(uint64_t)lane;
+    const uint64_t kprev = __shfl_up(key, 1, 64);
+    const uint64_t knext = __shfl_down(key, 1, 64);
+    int head = (lane == 0 || kprev != key) ? 1 : 0;
+    double sum = has ? v : 0.0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double so = __shfl_up(sum, d, 64);
+        const int ho = __shfl_up(head, d, 64);
+        if (lane >= d && !head) {
+            sum += so;
+            head = ho;
+        }
+    }
+    if (has && (lane == 63 || knext != key))
+        atomic_add_f64(j_base(P) + key, sum);
 }
 
 template <class C>
@@ -371,19 +416,37 @@ __device__ __forceinline__ void finish(const kparams &P, const ray_t &r, uint64_
     }
 }
 
-// Whole history in one thread (the integrator's launch shape).
-template <class C>
-__device__ void packet(const kparams &P, uint64_t slot, bool lds, C &c)
+// Whole histories, one thread per packet (the integrator's launch shape);
+// the wave steps its lanes together so deposit() sees all 64.
+template <class C> __device__ void history(const kparams &P, bool lds, C &c)
 {
-    const int band = band_of(P, slot);
-    const uint64_t pid = P.packet_offset + (slot - P.band_start[band]);
-    ray_t r;
-    start(P, r, band, pid);
-    int steps = P.step_max, status;
-    uint64_t ncross = 0;
-    while ((status = step(P, r, band, steps, ncross, lds)) == 0) {
+    const int lane = threadIdx.x & 63;
+    const bool runs = P.order != nullptr;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+         base < P.total; base += stride) {
+        const uint64_t k = base + lane;
+        ray_t r;
+        int band = 0, steps = P.step_max, status = -1;
+        uint64_t slot = 0, ncross = 0;
+        if (k < P.total) {
+            slot = slot_at(P, k);
+            band = band_of(P, slot);
+            start(P, r, band, P.packet_offset + (slot - P.band_start[band]));
+            status = 0;
+        }
+        while (__ballot(status == 0)) {
+            int64_t gc = -1;
+            double v = 0.0;
+            if (status == 0)
+                status = step(P, r, steps, ncross, gc, v);
+            deposit(P, lds, runs, gc >= 0, gc, band, v);
+            if (status > 0) {
+                finish(P, r, slot, ncross, status, c);
+                status = -1;
+            }
+        }
     }
-    finish(P, r, slot, ncross, status, c);
 }
 
 // Persistent lanes: one step per iteration, finished lanes refilled from a
@@ -396,6 +459,7 @@ template <class C> __device__ void persist(const kparams &P, bool lds, C &c)
     int mode = M_NEED, band = 0, steps = 0;
     uint64_t slot = 0, ncross = 0;
     const int lane = threadIdx.x & 63;
+    const bool runs = P.order != nullptr;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
     for (;;) {
@@ -425,7 +489,7 @@ template <class C> __device__ void persist(const kparams &P, bool lds, C &c)
             }
             if (mode == M_NEED) {
                 if (my < P.total) {
-                    slot = my;
+                    slot = slot_at(P, my);
                     band = band_of(P, slot);
                     start(P, r, band, P.packet_offset + (slot - P.band_start[band]));
                     steps = P.step_max;
@@ -436,10 +500,13 @@ template <class C> __device__ void persist(const kparams &P, bool lds, C &c)
                 }
             }
         }
-        if (mode != M_RUN)
-            continue;
-        const int status = step(P, r, band, steps, ncross, lds);
-        if (status != 0) {
+        int status = 0;
+        int64_t gc = -1;
+        double v = 0.0;
+        if (mode == M_RUN)
+            status = step(P, r, steps, ncross, gc, v);
+        deposit(P, lds, runs, gc >= 0, gc, band, v);
+        if (mode == M_RUN && status != 0) {
             finish(P, r, slot, ncross, status, c);
             mode = M_NEED;
         }

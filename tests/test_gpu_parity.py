@@ -19,7 +19,12 @@ pytestmark = pytest.mark.gpu
 
 
 SCHEDULES = [dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64),
-             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1)]
+             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1),
+             # packets marched in emission-direction order: same histories
+             dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64,
+                  coherent=1),
+             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1,
+                  coherent=1)]
 
 
 def _run_gpu(w, end_state=True, precision=None, sched=None):
@@ -70,7 +75,8 @@ def _amr_workload(n_packets, precision=_abi.KP_F64, model=_abi.KP_HG_RAYLEIGH):
     return configs.Workload("amr_test", mesh, kappa, albedo, g, (0.51, 0.49, 0.5), run)
 
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history", "persistent_coherent",
+                                                   "history_coherent"])
 def test_f64_config1_bit_exact(oracle_lib, sched):
     w = configs.config1(n_packets=20000)
     got = _run_gpu(w, sched=sched)
@@ -82,7 +88,8 @@ def test_f64_config1_bit_exact(oracle_lib, sched):
                                atol=1e-12 * np.abs(ref["image"]).max())
 
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history", "persistent_coherent",
+                                                   "history_coherent"])
 def test_f64_amr_bit_exact(oracle_lib, sched):
     w = _amr_workload(8000)
     got = _run_gpu(w, sched=sched)
@@ -164,7 +171,8 @@ def _periodic_workload(n_packets, precision=_abi.KP_F64):
     return configs.Workload("periodic_test", mesh, kappa, albedo, g, (1.37, 0.05, 0.5), run)
 
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history"])
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persistent", "history", "persistent_coherent",
+                                                   "history_coherent"])
 def test_f64_periodic_bit_exact(oracle_lib, sched):
     w = _periodic_workload(4000)
     got = _run_gpu(w, sched=sched)

@@ -18,7 +18,11 @@ SRC = (0.25, 0.15, 0.1)
 SCHEDULES = [dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64),
              dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=128, blocks_per_cu=2, chunk=16,
                   event_batch=1),
-             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1)]
+             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1),
+             dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64,
+                  coherent=1),
+             dict(kind=_abi.KP_SCHED_HISTORY, block_threads=256, blocks_per_cu=1, chunk=1,
+                  coherent=1)]
 
 
 def _shell_mesh():
@@ -60,7 +64,8 @@ def _check(got, ref):
     assert np.allclose(got["J"], ref["J"], rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("sched", SCHEDULES, ids=["persist256", "persist128_b1", "history"])
+@pytest.mark.parametrize("sched", SCHEDULES, ids=["persist256", "persist128_b1", "history",
+                                                   "persist_coherent", "history_coherent"])
 def test_sph_bit_exact_shell(sched):
     m = _shell_mesh()
     run = RunSpec(bands=[BandSpec(n_packets=20000, kappa_scale=0.0),
@@ -85,18 +90,21 @@ def test_sph_bit_exact_wedge_and_step_budget():
     assert np.any(st == _abi.KP_END_STEPS) and np.any(st == _abi.KP_END_ESCAPED)
 
 
-def test_sph_graph_replay_matches_forwards():
+@pytest.mark.parametrize("coherent", [0, 1])
+def test_sph_graph_replay_matches_forwards(coherent):
     import torch
 
     m = _shell_mesh()
-    run = RunSpec(bands=[BandSpec(n_packets=8000, kappa_scale=0.0)], seed=5, step_max=100000)
+    run = RunSpec(bands=[BandSpec(n_packets=8000, kappa_scale=0.0),
+                         BandSpec(n_packets=3000, kappa_scale=0.0)], seed=5, step_max=100000)
     eng = Engine(m)
+    eng.set_schedule(coherent=coherent)
     out_g = eng.alloc_outputs(run)
     g = eng.graph(SRC, run, out_g, n_chunks=2)
     g.launch()
     out_e = eng.alloc_outputs(run)
     eng.forward(SRC, run, out_e)
-    run2 = RunSpec(**{**run.__dict__, "packet_offset": 8000})
+    run2 = RunSpec(**{**run.__dict__, "packet_offset": 8000})  # chunk stride: max band count
     eng.forward(SRC, run2, out_e)
     torch.cuda.synchronize()
     assert torch.allclose(out_g["J"], out_e["J"], rtol=1e-12, atol=0)
