@@ -13,6 +13,8 @@
  * adds; "section|key" as in io/args/input.cpp):
  *   [mesh] x_min, x_max (3 reals), n_cell_global, n_cell_block (3 ints;
  *          cubic blocks) ......................... meshgen.cpp:13-40
+ *          coord (cartesian|spherical: meshgen::sph_t axes r, theta,
+ *          phi, spherical.h:9-140), r_ratio (1) .... cylindrical.h:18-32
  *   [boundary] kinds (6 words, "per..." = periodic) meshgen.cpp:47-52
  *   [refine_region*] level, x_min/x_max or loc ... meshgen.cpp:111-142
  *   [particle] n_hv (1) ........................... radiation.h:26
@@ -26,6 +28,11 @@
  *            centre of the domain), weight (1), incl (degrees, none),
  *            image_n (128), image_half (0.5), precision (f64|f32),
  *            rr_weight (0.01), rr_survive (0.1), schedule (persistent|history)
+ *   On a spherical mesh the photon streams (photon_sph.h): kappa_scale
+ *   defaults to 0, the source (a Cartesian point) to the centre of the
+ *   (r, theta, phi) box, precision must be f64, incl empty, and the engine's
+ *   one-thread-per-packet schedule in emission-direction order is kept unless
+ *   photon|schedule is given.
  */
 #ifndef KP_MODULE_H
 #define KP_MODULE_H

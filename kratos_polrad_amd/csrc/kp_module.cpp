@@ -221,6 +221,9 @@ struct kp_module {
     std::array<double, 3> x_min{0, 0, 0}, x_max{1, 1, 1};
     std::array<int, 3> n_global{0, 0, 0}, n_block{0, 0, 0};
     std::array<int, 3> periodic{0, 0, 0};
+    int coord = KP_COORD_CARTESIAN;
+    double r_ratio = 1.0;
+    bool sched_given = false;
     // [particle]
     int n_hv = 1, n_cyc = 1, n_th = 64, step_max = 100;
     int64_t n_par = 1;
@@ -270,6 +273,13 @@ struct kp_module {
         }
         if (n_block[0] != n_block[1] || n_block[0] != n_block[2] || (n_block[0] & 1))
             throw kp::error(KP_ERR_UNSUPPORTED, "ini: cubic blocks of an even cell count only");
+        // meshgen::sph_t (spherical.h) and its [mesh] r_ratio (cylindrical.h:21-23)
+        const std::string cs = ini.get<std::string>("mesh", "coord", "cartesian");
+        if (cs == "spherical")
+            coord = KP_COORD_SPHERICAL;
+        else if (cs != "cartesian")
+            throw kp::error(KP_ERR_ARG, "ini: mesh|coord must be cartesian or spherical");
+        r_ratio = ini.get<double>("mesh", "r_ratio", 1.0);
         const auto kinds = ini.list<std::string>("boundary", "kinds");
         for (size_t s = 0; s < kinds.size() && s < 6; ++s)
             if (kinds[s].find("per") != std::string::npos)
@@ -293,6 +303,7 @@ struct kp_module {
         precision = pr == "f32" ? KP_F32 : KP_F64;
         const std::string sk = ini.get<std::string>("photon", "schedule", "persistent");
         sched_kind = sk == "history" ? KP_SCHED_HISTORY : KP_SCHED_PERSISTENT;
+        sched_given = ini.has("photon", "schedule");
         p_max = ini.get<double>("photon", "p_max", 1.0);
         kappa = ini.get<double>("photon", "kappa", 0.0);
         weight = ini.get<double>("photon", "weight", 1.0);
@@ -311,10 +322,20 @@ struct kp_module {
         };
         albedo = per_band("albedo", 1.0);
         g = per_band("g", 0.0);
-        kappa_scale = per_band("kappa_scale", 1.0);
+        // the spherical photon streams (photon_sph.h): no extinction by default
+        kappa_scale = per_band("kappa_scale", coord == KP_COORD_SPHERICAL ? 0.0 : 1.0);
         incl = ini.list<double>("photon", "incl");
         for (int a = 0; a < 3; ++a)
             source[a] = 0.5 * (x_min[a] + x_max[a]);
+        if (coord == KP_COORD_SPHERICAL) {
+            // the centre of the (r, theta, phi) box as a Cartesian point
+            const double r = source[0], th = source[1], ph = source[2];
+            source = {r * std::sin(th) * std::cos(ph), r * std::sin(th) * std::sin(ph),
+                      r * std::cos(th)};
+            if (!incl.empty() || precision != KP_F64)
+                throw kp::error(KP_ERR_UNSUPPORTED, "ini: spherical meshes stream in f64 "
+                                                    "without observers (photon|incl)");
+        }
         source = ini.arr<double, 3>("photon", "source", false, source);
         image_n = ini.get<int>("photon", "image_n", 128);
         image_half = ini.get<double>("photon", "image_half", 0.5);
@@ -332,6 +353,11 @@ struct kp_module {
         kp_status st = kp_mesh_create(root, n_cell, x_min.data(), x_max.data(), per, &mesh);
         if (st != KP_OK)
             throw kp::error(st, kp_last_error());
+        if (coord != KP_COORD_CARTESIAN) {
+            st = kp_mesh_set_coord(mesh, coord, r_ratio);
+            if (st != KP_OK)
+                throw kp::error(st, kp_last_error());
+        }
         // load_refinement: every [refine_region*] section, in file order
         for (const auto &sec : ini.sections) {
             if (sec.find("refine_region") == std::string::npos)
@@ -360,8 +386,10 @@ struct kp_module {
             throw kp::error(st, kp_last_error());
         kp_schedule s;
         kp_get_schedule(eng, &s);
-        s.kind = sched_kind;
-        if (sched_kind == KP_SCHED_HISTORY)
+        // spherical engines keep their own default unless the input names one
+        if (coord == KP_COORD_CARTESIAN || sched_given)
+            s.kind = sched_kind;
+        if (s.kind == KP_SCHED_HISTORY)
             s.block_threads = 256;
         st = kp_set_schedule(eng, &s);
         if (st != KP_OK)

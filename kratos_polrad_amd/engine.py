@@ -96,7 +96,10 @@ class Engine:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _abi.lib().kp_engine_destroy(h)
+            try:
+                _abi.lib().kp_engine_destroy(h)
+            except (AttributeError, TypeError):
+                pass  # interpreter shutdown: module globals are already gone
             self._h = None
 
     @property
@@ -234,5 +237,8 @@ class Graph:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _abi.lib().kp_graph_destroy(h)
+            try:
+                _abi.lib().kp_graph_destroy(h)
+            except (AttributeError, TypeError):
+                pass  # interpreter shutdown: module globals are already gone
             self._h = None

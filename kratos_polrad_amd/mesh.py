@@ -44,7 +44,10 @@ class Mesh:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _abi.lib().kp_mesh_destroy(h)
+            try:
+                _abi.lib().kp_mesh_destroy(h)
+            except (AttributeError, TypeError):
+                pass  # interpreter shutdown: module globals are already gone
             self._h = None
 
     @property

@@ -23,7 +23,10 @@ class PhotonModule:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _abi.lib().kp_module_destroy(h)
+            try:
+                _abi.lib().kp_module_destroy(h)
+            except (AttributeError, TypeError):
+                pass  # interpreter shutdown: module globals are already gone
             self._h = None
 
     def init(self, device: int = 0) -> "PhotonModule":

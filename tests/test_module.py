@@ -174,3 +174,71 @@ def test_step_equals_forward_runs(tmp_path):
         np.testing.assert_array_equal(f, got["J"][b * n3:(b + 1) * n3].ravel())
     img = np.frombuffer(c["photon|image"][0], dtype=np.float64)
     np.testing.assert_array_equal(img, got["image"].ravel())
+
+
+SPH_INI = """
+[mesh]
+coord = spherical
+r_ratio = -1
+x_min = 0.1 0 0
+x_max = 1.0 3.141592653589793 6.283185307179586
+n_cell_global = 32 16 32
+n_cell_block = 8 8 8
+
+[boundary]
+kinds = outflow outflow outflow outflow periodic periodic
+
+[refine_region_src]
+level = 1
+x_min = 0.2 1.0 0.2
+x_max = 0.4 1.6 0.9
+
+[particle]
+n_hv = 1
+n_par = 5000
+n_cycle_stp = 2
+step_max = 100000
+
+[photon]
+seed = 3
+source = 0.25 0.15 0.1
+"""
+
+
+def test_spherical_ini_builds_the_sph_mesh():
+    m = PhotonModule(SPH_INI).init(device=-1)
+    i = m.info()
+    ref = Mesh(root_blocks=(4, 2, 4), n_cell=8, x_min=(0.1, 0.0, 0.0),
+               x_max=(1.0, np.pi, 2 * np.pi), periodic=(0, 0, 1), coord="spherical",
+               r_ratio=-1.0)
+    ref.refine_region(1, (0.2, 1.0, 0.2), (0.4, 1.6, 0.9))
+    ref.finalize()
+    assert (i["n_blocks"], i["n_cells"]) == (ref.n_blocks, ref.n_cells)
+    with pytest.raises(_abi.KPError):
+        PhotonModule(SPH_INI.replace("seed = 3", "seed = 3\nincl = 90"))
+    with pytest.raises(_abi.KPError):
+        PhotonModule(SPH_INI.replace("coord = spherical", "coord = polar"))
+
+
+@pytest.mark.gpu
+def test_spherical_step_equals_forward_runs():
+    import torch
+
+    from kratos_polrad_amd import configs
+
+    m = PhotonModule(SPH_INI).init(device=0)
+    m.step()
+    got = m.tallies()
+    mesh = configs.sph_mesh(root_blocks=(4, 2, 4), n_cell=8, r_lim=(0.1, 1.0), r_ratio=-1.0,
+                            refine=((1, (0.2, 1.0, 0.2), (0.4, 1.6, 0.9)),))
+    eng = Engine(mesh)
+    out = None
+    for c in range(2):
+        run = RunSpec(bands=[BandSpec(n_packets=5000, kappa_scale=0.0)], seed=3,
+                      packet_offset=5000 * c, step_max=100000)
+        out = out or eng.alloc_outputs(run)
+        eng.forward((0.25, 0.15, 0.1), run, out)
+    torch.cuda.synchronize()
+    J = out["J"].cpu().numpy().reshape(-1, 1)
+    np.testing.assert_allclose(got["J"], J, rtol=1e-12, atol=1e-12 * np.abs(J).max())
+    assert np.array_equal(got["counters"], out["counters"].cpu().numpy().astype(np.uint64))
