@@ -1256,13 +1256,15 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
             };
             const bool sph_mesh = e->coord == KP_COORD_SPHERICAL;
             // schedule kind and packet order together: lockstep histories pay
-            // off in emission-direction order, which persistent lanes lose.
-            // With observers the one-thread-per-packet kernels march every
+            // off in emission-direction order; persistent lanes refill at
+            // different times, lose the lockstep and then only collide on
+            // the same J addresses, so that pair is not a candidate.  With
+            // observers the one-thread-per-packet kernels march every
             // peel-off ray inside the history, unbatched: not a candidate.
             for (int kind : {KP_SCHED_PERSISTENT, KP_SCHED_HISTORY})
                 for (int coh : {0, 1}) {
                     if ((cur.kind == kind && cur.coherent == coh) ||
-                        (coh && total >= (1ull << 31)) ||
+                        (coh && (total >= (1ull << 31) || kind == KP_SCHED_PERSISTENT)) ||
                         (kind == KP_SCHED_HISTORY && run->n_incl > 0))
                         continue;
                     kp_schedule c = cur;
