@@ -27,6 +27,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line per finished stage on stderr (stdout carries only the JSON)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -265,6 +271,8 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
                 eng.set_media(kap, w.albedo, w.g)
                 for b in range(n_batches):
                     fd[b, k] += sgn * loss(b)[0] / step
+            if (k + 1) % 16 == 0:
+                progress(f"finite differences (eps {e:g}): {k + 1} of {n_cells} cells")
         return fd
 
     def compare(fd):
@@ -332,6 +340,7 @@ def large(packets=1_000_000_000, chunk=100_000_000):
     eng = Engine(w.mesh)
     eng.set_media(w.kappa, w.albedo, w.g)
     t_setup = time.perf_counter() - t0
+    progress(f"large workload ready ({w.mesh.n_blocks} blocks)")
     res = {"workload": w.name, "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
            "levels": int(w.mesh.level.max()) + 1, "packets": packets, "chunk": chunk,
            "n_incl": len(w.run.incl_deg), "image_n": w.run.image_n, "setup_s": t_setup}
@@ -349,6 +358,7 @@ def large(packets=1_000_000_000, chunk=100_000_000):
                     "packets_done": float(cnt[7]), "crossings": float(cnt[0] + cnt[1])}
     img32 = out["image"].cpu().numpy()
     del g, out
+    progress(f"large: {packets:.3g} packets in {wall:.1f} s")
     # FP64 and mixed on the same first chunk
     runs = {}
     for prec in (_abi.KP_F64, _abi.KP_F32):
@@ -363,6 +373,7 @@ def large(packets=1_000_000_000, chunk=100_000_000):
         c = o["counters"].cpu().numpy().astype(np.float64)
         runs[prec] = (dt, (c[0] + c[1]) / dt, o["image"].cpu().numpy())
         del o
+        progress(f"large: one chunk, precision {prec}, {dt:.1f} s")
     res["fp64_one_chunk"] = {"seconds": runs[_abi.KP_F64][0], "value": runs[_abi.KP_F64][1]}
     res["mixed_one_chunk"] = {"seconds": runs[_abi.KP_F32][0], "value": runs[_abi.KP_F32][1]}
     res["speedup_mixed_vs_fp64"] = runs[_abi.KP_F32][1] / runs[_abi.KP_F64][1]
@@ -580,7 +591,9 @@ def main():
         img = out["image"].cpu().numpy()[0, 0]
         return wall, kern, cnt, img
 
+    progress(f"workload {w.name} ready ({w.mesh.n_blocks} blocks)")
     wall32, kern32, cnt32, img32 = timed(_abi.KP_F32, args.steps, args.warmup)
+    progress("mixed-precision steps done")
     sched32 = eng.get_schedule()
     cross32 = cnt32[0] + cnt32[1]
     value = cross32 / wall32
@@ -626,23 +639,31 @@ def main():
             "fp64_vs_independent_fp64": img_err(img64, ref),
         }
 
+    if not args.skip_f64:
+        progress("FP64 steps done")
     if not args.skip_batched:
         res["batched"] = batched(args.steps, args.batched_scale)
+        progress("64-band batched run done")
 
     if not args.skip_adjoint:
         res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))
+        progress("adjoint timing done")
         if args.fd_cells > 0:
             res["adjoint"]["fd_check"] = adjoint_fd(int(args.adjoint_packets), args.fd_cells)
+            progress("finite-difference check done")
 
     if not args.skip_compaction:
         res["compaction"] = compaction()
+        progress("compaction done")
 
     if not args.skip_large:
         res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000))
+        progress("large run done")
 
     if not args.skip_sph:
         res["spherical"] = spherical(args.steps, args.warmup, args.sph_packets,
                                      args.cpu_seconds / 2, args.no_cpu)
+        progress("spherical run done")
 
     if not args.no_cpu:
         import oracle

@@ -260,7 +260,8 @@ typedef struct kp_schedule {
      * equal-cell tallies of neighbouring lanes are summed before one atomic.
      * Histories are unchanged (end_state stays indexed by slot); tallies
      * differ by summation order only.  Takes about 26 B of scratch per
-     * packet; launches of 2^31 packets or more run in slot order.  0 = slot
+     * packet; launches of 2^31 packets or more, or of more than 2^22 bands,
+     * run in slot order.  0 = slot
      * order.  Engines on spherical meshes start with the one-thread-per-packet
      * schedule in this order. */
     int32_t coherent;

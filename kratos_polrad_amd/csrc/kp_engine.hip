@@ -701,11 +701,13 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
     }
     // emission-direction order: key and permutation arrays and the sort
     // workspace, grown to this launch
-    pl.coherent = S.coherent != 0 && total < (1ull << 31); // else slot order
-    if (pl.coherent) {
+    // slot order instead when the launch or the band count is too large for
+    // the 31-bit sort keys (fewer than 4 direction bits per axis left)
+    pl.coherent = S.coherent != 0 && total < (1ull << 31);
+    if (pl.coherent)
         pl.dir_bits = dir_key_bits(run->n_bands, pl.key_bits);
-        if (pl.dir_bits < 4)
-            throw kp::error(KP_ERR_UNSUPPORTED, "coherent order supports up to 2^22 bands");
+    pl.coherent = pl.coherent && pl.dir_bits >= 4;
+    if (pl.coherent) {
         if (fb.ord_cap < total) {
             dfree(fb.keys);
             dfree(fb.order);
