@@ -1,0 +1,154 @@
+"""Size-independent properties at the benchmark workloads' full mesh sizes.
+
+The CPU restatement pins the kernels on small meshes; on the full meshes of
+the benchmark (config 2: 2,234 blocks / 9.15e6 cells; config 5: 28,435 blocks /
+1.2e8 cells; the spherical shell: 1,402 blocks / 5.7e6 cells) these check, with
+reduced packet counts, what must hold whatever the size:
+  - FP64 histories do not depend on the schedule or on the packet order
+    (persistent / one thread per packet, slot / emission-direction order):
+    every end state bit-identical, tallies equal to the order of FP64 sums;
+  - two chunks with consecutive packet ids add up to one launch of both;
+  - the mixed path tracks the FP64 path on the same packets (same random
+    numbers, FP32 block-local transport): J and the images agree far inside
+    the Monte Carlo noise;
+  - every packet is accounted for (escaped, roulette, step budget), none lost;
+  - on the spherical shell, per root-level radial shell, sum(J V) equals the
+    exact in-shell length of the packets' straight rays (from the end-state
+    directions) divided by the 1.001 overshoot.
+"""
+import numpy as np
+import pytest
+
+from kratos_polrad_amd import _abi, configs
+from kratos_polrad_amd.engine import Engine, RunSpec
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def config2():
+    return configs.config2(n_packets=200_000, precision=_abi.KP_F64)
+
+
+def _run(eng, w, run, end_state=False, J=True):
+    import torch
+
+    out = eng.alloc_outputs(run, end_state=end_state, J=J)
+    eng.forward(w.source, run, out)
+    torch.cuda.synchronize()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["counters"] = res["counters"].astype(np.uint64)
+    return res
+
+
+def test_config2_fp64_histories_independent_of_schedule_and_order(config2):
+    w = config2
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    ref = None
+    for kind in (_abi.KP_SCHED_PERSISTENT, _abi.KP_SCHED_HISTORY):
+        for coh in (0, 1):
+            eng.set_schedule(kind=kind, block_threads=256, blocks_per_cu=4, chunk=64,
+                             coherent=coh)
+            got = _run(eng, w, w.run, end_state=True)
+            if ref is None:
+                ref = got
+                continue
+            assert np.array_equal(got["end_state"], ref["end_state"]), (kind, coh)
+            assert np.array_equal(got["counters"], ref["counters"])
+            np.testing.assert_allclose(got["J"], ref["J"], rtol=1e-12, atol=1e-12 * ref["J"].max())
+            np.testing.assert_allclose(got["image"], ref["image"], rtol=1e-11,
+                                       atol=1e-12 * np.abs(ref["image"]).max())
+    c = ref["counters"]
+    assert c[_abi.KP_C_LOST] == 0
+    assert c[_abi.KP_C_ESCAPED] + c[_abi.KP_C_KILLED_RR] + c[_abi.KP_C_KILLED_STEPS] == 200_000
+
+
+def test_config2_chunks_add_up_to_one_launch(config2):
+    w = config2
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    half = RunSpec(**{**w.run.__dict__, "bands": [type(w.run.bands[0])(
+        **{**w.run.bands[0].__dict__, "n_packets": 100_000})]})
+    a = _run(eng, w, half)
+    b = _run(eng, w, RunSpec(**{**half.__dict__, "packet_offset": 100_000}))
+    both = _run(eng, w, w.run)
+    np.testing.assert_allclose(a["J"] + b["J"], both["J"], rtol=1e-11,
+                               atol=1e-12 * both["J"].max())
+    np.testing.assert_allclose(a["image"] + b["image"], both["image"], rtol=1e-10,
+                               atol=1e-12 * np.abs(both["image"]).max())
+    assert np.array_equal(a["counters"] + b["counters"], both["counters"])
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_config2_mixed_tracks_fp64_on_the_same_packets():
+    w = configs.config2(n_packets=2_000_000, precision=_abi.KP_F64)
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    r64 = _run(eng, w, w.run)
+    r32 = _run(eng, w, RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F32}))
+    assert r32["counters"][_abi.KP_C_LOST] == 0
+    assert r32["counters"][_abi.KP_C_PACKETS] == 2_000_000
+    # same random numbers: far below the Monte Carlo noise of the image
+    # (the image error against independent FP64 packets is ~7e-4 at 1e8)
+    assert _rel(r32["J"], r64["J"]) < 2e-3
+    for k in range(3):
+        assert _rel(r32["image"][..., k], r64["image"][..., k]) < 2e-3
+
+
+def test_config5_full_mesh_accounting_and_tracking():
+    w = configs.config5(n_packets=1_000_000, precision=_abi.KP_F32)
+    assert w.mesh.n_blocks > 25_000
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    r32 = _run(eng, w, w.run, J=False)
+    run64 = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+    r64 = _run(eng, w, run64, J=False)
+    ind = _run(eng, w, RunSpec(**{**run64.__dict__, "packet_offset": 1 << 40}), J=False)
+    for r in (r32, r64, ind):
+        c = r["counters"]
+        assert c[_abi.KP_C_LOST] == 0
+        assert c[_abi.KP_C_ESCAPED] + c[_abi.KP_C_KILLED_RR] + c[_abi.KP_C_KILLED_STEPS] == 1_000_000
+    # the optically thick disk lets a few FP32 histories part from their FP64
+    # twins; what parts must stay well inside the Monte Carlo noise, measured
+    # by an independent FP64 run of the same size
+    for o in range(len(w.run.incl_deg)):
+        same = _rel(r32["image"][0, o, ..., 0], r64["image"][0, o, ..., 0])
+        noise = _rel(ind["image"][0, o, ..., 0], r64["image"][0, o, ..., 0])
+        assert same < 0.25 * noise, (o, same, noise)
+
+
+def test_spherical_shell_full_mesh_shell_lengths():
+    n = 100_000
+    w = configs.config_sph(n_packets=n)
+    eng = Engine(w.mesh)
+    r = _run(eng, w, w.run, end_state=True)
+    c = r["counters"]
+    assert c[_abi.KP_C_ESCAPED] == n and c[_abi.KP_C_LOST] == 0
+    m = w.mesh
+    V = m.cell_volumes()
+    r_f = np.unique(np.round(m.xf[m.level == 0][:, 0, :].ravel(), 13))
+    shell = np.clip(np.searchsorted(r_f, m.cell_centers()[:, 0]) - 1, 0, len(r_f) - 2)
+    got = np.bincount(shell, weights=r["J"] * V, minlength=len(r_f) - 1)
+    # exact in-shell lengths of the straight rays x0 + t d, t in [0, chord]
+    x0 = np.array(w.source)
+    d = r["end_state"][:, _abi.KP_ES_D:_abi.KP_ES_D + 3]
+    b = d @ x0
+    c_in = x0 @ x0 - r_f[0] ** 2
+    det_in = b * b - c_in
+    t_in = np.where(det_in > 0, -b - np.sqrt(np.maximum(det_in, 0)), np.inf)
+    t_out = -b + np.sqrt(b * b - (x0 @ x0 - r_f[-1] ** 2))
+    chord = np.where(t_in > 0, t_in, t_out)
+    # length of [0, chord] inside the ball of radius R, for every face radius
+    def inside(R):
+        det = b * b - (x0 @ x0 - R * R)
+        s = np.sqrt(np.maximum(det, 0.0))
+        lo = np.clip(-b - s, 0.0, chord)
+        hi = np.clip(-b + s, 0.0, chord)
+        return np.where(det > 0, hi - lo, 0.0)
+    ball = np.array([inside(R).sum() for R in r_f])
+    want = np.diff(ball) / 1.001
+    assert np.allclose(got, want, rtol=2e-3, atol=1e-9 * want.max()), np.max(np.abs(got / want - 1))
