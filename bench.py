@@ -30,6 +30,18 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 T_START = time.perf_counter()
 
 
+def cpu_model():
+    """The host CPU model (the baseline's cores are this machine's)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def progress(msg):
     """One line per finished stage on stderr (stdout carries only the JSON)."""
     print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
@@ -527,7 +539,7 @@ def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
             "value": float(o["counters"][_abi.KP_C_CROSS_MAIN] / dt), "unit": "cell-crossings/s",
             "cores": cores, "kind": "port",
             "sample": f"{npk} packets of the same workload (FP64 spherical oracle, "
-                      f"pthreads), {dt:.1f} s"}
+                      f"pthreads), {dt:.1f} s on {cores} threads of {cpu_model()}"}
     del out, eng
     return res
 
@@ -692,7 +704,7 @@ def main():
             "value": float((c[0] + c[1]) / dt), "unit": "cell-crossings/s", "cores": cores,
             "kind": "port",
             "sample": f"{n_cpu} packets of the same workload (FP64 oracle, pthreads), "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s on {cores} threads of {cpu_model()}"}
     print(json.dumps(res))
 
 
