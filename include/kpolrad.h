@@ -265,6 +265,12 @@ typedef struct kp_schedule {
      * order.  Engines on spherical meshes start with the one-thread-per-packet
      * schedule in this order. */
     int32_t coherent;
+    /* kp_adjoint, mixed precision, persistent: 1 (default) = the replay
+     * queues its peel-off rays (a 32 B record each) and a separate kernel
+     * marches them in image-plane order, lanes in lockstep, summing equal-cell
+     * gradient deposits of neighbouring lanes before one atomic; 0 = the
+     * replay marches them itself.  Same deposits, different summation order. */
+    int32_t peel_queue;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

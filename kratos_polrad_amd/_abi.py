@@ -106,6 +106,7 @@ class Schedule(C.Structure):
         ("event_batch", C.c_int32),
         ("waves_per_simd", C.c_int32),
         ("coherent", C.c_int32),
+        ("peel_queue", C.c_int32),
     ]
 
 

@@ -20,6 +20,17 @@ struct dband {
     uint32_t pad;
 };
 
+// A peel-off ray of an adjoint replay whose gradient deposits are marched
+// by a separate kernel: start point (block-local FP32), block, cell, observer,
+// band and its A_k.
+struct peel_rec {
+    float x[3];
+    int32_t ib;
+    uint32_t cell_o; // i0 | i1 << 8 | i2 << 16 | observer << 24
+    int32_t band;
+    double A;
+};
+
 // Everything a transport launch needs, passed by value.
 struct kparams {
     // mesh (block table, FP64)
@@ -95,6 +106,10 @@ struct kparams {
     uint32_t adj_cap;
     uint64_t adj_lanes;
     double *jlin;                  // device scalar
+    peel_rec *pq;                  // queue of pass-2 peel-off rays (or null)
+    unsigned int *pq_count;
+    uint32_t pq_cap;
+    const int32_t *pq_order;       // queue slots in image-plane order
 };
 
 __device__ __forceinline__ double *g_base(const kparams &P, double *base)
@@ -143,6 +158,32 @@ __device__ __forceinline__ int band_of(const kparams &P, uint64_t slot)
             hi = mid - 1;
     }
     return lo;
+}
+
+// base[idx] += v for every lane with `has`, called by all 64 lanes of the
+// wave together.  Runs of neighbouring lanes with the same idx are summed by
+// a segmented scan across the wave first and the last lane of each run
+// issues one atomic: FP64 atomics on one address serialise in the L2, and
+// lanes that march neighbouring rays in lockstep share cells.
+__device__ __forceinline__ void wave_run_add(double *base, bool has, uint64_t idx, double v)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t key = has ? idx : ~(uint64_t)lane; // lanes without a value: own keys
+    const uint64_t kprev = __shfl_up(key, 1, 64);
+    const uint64_t knext = __shfl_down(key, 1, 64);
+    int head = (lane == 0 || kprev != key) ? 1 : 0;
+    double sum = has ? v : 0.0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double so = __shfl_up(sum, d, 64);
+        const int ho = __shfl_up(head, d, 64);
+        if (lane >= d && !head) {
+            sum += so;
+            head = ho;
+        }
+    }
+    if (has && (lane == 63 || knext != key))
+        atomic_add_f64(base + key, sum);
 }
 
 // Dynamic LDS of persistent forward launches: the workgroup's accumulators,

@@ -71,7 +71,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0, 1};
     int n_cu = 0;
     fwd_bufs fb;                        // forward launch scratch
     // adjoint: per-thread rows of peel-off contributions, gradient replicas
@@ -79,6 +79,13 @@ struct kp_engine {
     size_t adj_cap = 0;
     double *g_rep = nullptr;
     size_t g_rep_cap = 0;
+    // mixed adjoint: queue of pass-2 peel-off rays, its sort keys, order and
+    // workspace
+    kp::peel_rec *pq = nullptr;
+    unsigned int *pq_count = nullptr;
+    int32_t *pq_keys = nullptr, *pq_order = nullptr;
+    void *pq_ws = nullptr;
+    size_t pq_cap = 0, pq_ws_cap = 0;
     std::map<std::string, kp_schedule> tuned; // autotuner cache
 };
 
@@ -359,6 +366,123 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
     return k_persist<kp::pol32, 4, ADJ>; // 128 VGPRs, no spills
 }
 
+__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
+
+// Sort key of every queued peel-off ray: observer, then the Morton code of
+// its start point's 2^9 x 2^9 image-plane cell, then its depth along the line
+// of sight (2^9 steps); unused slots sort last.
+__global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
+{
+    const unsigned int n = min(*P.pq_count, P.pq_cap);
+    double D = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        D += (P.xmax[a] - P.xmin[a]) * (P.xmax[a] - P.xmin[a]);
+    D = 0.5 * sqrt(D);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.pq_cap; k += stride) {
+        if (k >= n) {
+            keys[k] = (int32_t)((1u << 30) - 1u);
+            continue;
+        }
+        const kp::peel_rec rec = P.pq[k];
+        if (rec.ib < 0) { // unused rest of a wave's chunk
+            keys[k] = (int32_t)((1u << 30) - 1u);
+            continue;
+        }
+        const int o = (int)(rec.cell_o >> 24);
+        kp::f32::ray_t t;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            t.x[a] = rec.x[a];
+        t.ib = rec.ib;
+        float g[3];
+        kp::f32::global_pos(P, t, g);
+        double uh = 0.0, uv = 0.0, dz = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double r = (double)g[a] - P.center[a];
+            uh += r * P.eh[o][a];
+            uv += r * P.ev[o][a];
+            dz += r * P.no[o][a];
+        }
+        auto q9 = [](double f) { return (uint32_t)fmin(fmax(f * 512.0, 0.0), 511.0); };
+        const uint32_t u = q9((uh + P.image_half) / (2.0 * P.image_half));
+        const uint32_t v = q9((uv + P.image_half) / (2.0 * P.image_half));
+        const uint32_t z = q9((dz + D) / (2.0 * D));
+        uint32_t m = 0;
+        for (int b = 0; b < 9; ++b)
+            m |= (((u >> b) & 1u) << (2 * b)) | (((v >> b) & 1u) << (2 * b + 1));
+        keys[k] = (int32_t)(((uint32_t)o << 27) | (m << 9) | z);
+    }
+}
+
+// Gradient deposits of the queued peel-off rays, d/dlog(rho_c) += -dtau_c A_k
+// for every crossing: the same FP32 march, optical depth and cut-off as the
+// replay's own peel-off march, one thread per ray in image-plane order.
+__global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
+{
+    const unsigned int n = min(*P.pq_count, P.pq_cap);
+    const int lane = threadIdx.x & 63;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    double *gl = kp::g_base(P, P.g_logrho);
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+         base < P.pq_cap; base += stride) {
+        const uint64_t k = base + lane;
+        int64_t idx = -1;
+        if (k < P.pq_cap) {
+            const uint32_t s = (uint32_t)P.pq_order[k];
+            if (s < n && P.pq[s].ib >= 0)
+                idx = s;
+        }
+        // used slots sort first: a wave with none ends this thread's share
+        if (!__ballot(idx >= 0))
+            break;
+        kp::f32::ray_t r;
+        float ks = 0.0f;
+        double A = 0.0, tau = 0.0;
+        bool live = idx >= 0;
+        if (live) {
+            const kp::peel_rec rec = P.pq[idx];
+            const int o = (int)(rec.cell_o >> 24);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                r.x[a] = rec.x[a];
+                r.d[a] = (float)P.no[o][a];
+            }
+            r.i[0] = (int)(rec.cell_o & 255u);
+            r.i[1] = (int)((rec.cell_o >> 8) & 255u);
+            r.i[2] = (int)((rec.cell_o >> 16) & 255u);
+            r.ib = rec.ib;
+            kp::pol32::relocate(P, r);
+            ks = (float)P.bands[rec.band].kappa_scale;
+            A = rec.A;
+        }
+        while (__ballot(live)) {
+            bool has = false;
+            int64_t gc = 0;
+            double v = 0.0;
+            if (live) {
+                float l;
+                const int ap = kp::pol32::dl(r, l);
+                if (ap < 0) {
+                    live = false;
+                } else {
+                    gc = (int64_t)r.ib * P.n3 + kp::pol32::cell(P, r);
+                    const float kap = P.kappa[gc] * ks;
+                    const float dtau = kap * l;
+                    tau = tau + (double)dtau;
+                    has = true;
+                    v = -((double)dtau * A);
+                    if (tau > kp::kTauCut || kp::pol32::cross(P, r, ap, l) != kp::f64::ST_IN)
+                        live = false;
+                }
+            }
+            kp::wave_run_add(gl, has, (uint64_t)gc, v);
+        }
+    }
+}
+
 // One thread per packet, both replay passes in that thread (FP64).
 __global__ void __launch_bounds__(256) k_adjoint(kp::kparams P, kp::adj::aparams Q)
 {
@@ -411,6 +535,12 @@ void free_all(kp_engine *e)
     e->adj_cap = 0;
     dfree(e->g_rep);
     e->g_rep_cap = 0;
+    dfree(e->pq);
+    dfree(e->pq_count);
+    dfree(e->pq_keys);
+    dfree(e->pq_order);
+    dfree(e->pq_ws);
+    e->pq_cap = e->pq_ws_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
     e->stage = nullptr;
@@ -1059,10 +1189,51 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         P.work = e->fb.work;
         P.chunk = (uint32_t)S.chunk;
         P.batch = (uint32_t)S.event_batch;
+        // mixed precision: pass-2 peel-off rays go to a queue marched in
+        // image-plane order after the replay (capacity 4 rays per packet and
+        // observer; beyond it the replay marches them itself)
+        const bool queue = run->precision == KP_F32 && S.peel_queue && grads->d_logrho &&
+                           e->n_cell < 256;
+        if (queue) {
+            const uint64_t want_q = std::min<uint64_t>(
+                (1ull << 31) - 1, std::max<uint64_t>(1ull << 20, 4ull * total * run->n_incl));
+            const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
+            if (e->pq_cap < want_q) {
+                dfree(e->pq);
+                dfree(e->pq_keys);
+                dfree(e->pq_order);
+                e->pq_cap = 0;
+                e->pq = dmalloc<kp::peel_rec>(want_q);
+                e->pq_keys = dmalloc<int32_t>(want_q);
+                e->pq_order = dmalloc<int32_t>(want_q);
+                e->pq_cap = want_q;
+            }
+            if (e->pq_ws_cap < wsb) {
+                dfree(e->pq_ws);
+                e->pq_ws_cap = 0;
+                KP_HIP(hipMalloc(&e->pq_ws, wsb));
+                e->pq_ws_cap = wsb;
+            }
+            if (!e->pq_count)
+                e->pq_count = dmalloc<unsigned int>(1);
+            P.pq = e->pq;
+            P.pq_count = e->pq_count;
+            P.pq_cap = (uint32_t)e->pq_cap;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, e->pq_count);
+        }
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
         hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
                            dim3((unsigned)blocks), dim3(T), 0, stream, P);
         KP_HIP(hipGetLastError());
+        if (queue) {
+            hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, P, e->pq_keys);
+            kp::sort_order(e->pq_keys, e->pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
+                           stream);
+            P.pq_order = e->pq_order;
+            hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0, stream,
+                               P);
+            KP_HIP(hipGetLastError());
+        }
         if (P.g_nrep > 1) {
             if (gl)
                 hipLaunchKernelGGL(k_fold, dim3(1024), dim3(256), 0, stream, gl, P.g_logrho, nc,
@@ -1325,6 +1496,8 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: replicas must be in [0,256]");
         if (s->coherent != 0 && s->coherent != 1)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: coherent must be 0 or 1");
+        if (s->peel_queue != 0 && s->peel_queue != 1)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: peel_queue must be 0 or 1");
         e->sched = *s;
     });
 }

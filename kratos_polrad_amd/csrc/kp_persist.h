@@ -226,8 +226,72 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
 }
 
 // Start the peel-off ray toward observer L.o from the saved main position.
+// Wave-private chunk [base, end) of the peel-off queue, in LDS (one entry per
+// wave of the workgroup): a wave claims kPqChunk slots with one atomic and
+// fills them itself, so the global counter sees one atomic per chunk.
+constexpr uint32_t kPqChunk = 256;
+__shared__ uint32_t kp_pq_base[4], kp_pq_end[4];
+
+// Queue the current pass-2 peel-off ray (start point L.sx / L.si / L.sib,
+// observer L.o, A_k = L.Acur) for the gradient-march kernel.  Called by the
+// lanes that queue, together.  Returns false beyond the queue's capacity
+// (the lane then marches the ray itself).
+template <class Pol>
+__device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &L)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t n = (uint32_t)__popcll(m);
+    const uint32_t rank = (uint32_t)__popcll((lane == 0) ? 0ull : (m & (~0ull >> (64 - lane))));
+    const uint32_t b0 = kp_pq_base[w], e0 = kp_pq_end[w];
+    const uint32_t avail = e0 - b0;
+    uint64_t k = (uint64_t)b0 + rank;
+    if (n > avail) {
+        // the rest of the chunk, then a new one
+        unsigned int nb = 0;
+        if (lane == leader)
+            nb = atomicAdd(P.pq_count, kPqChunk);
+        nb = __shfl(nb, leader, 64);
+        if (rank >= avail)
+            k = (uint64_t)nb + (rank - avail);
+        if (lane == leader) {
+            kp_pq_base[w] = nb + (n - avail);
+            kp_pq_end[w] = nb + kPqChunk;
+        }
+    } else if (lane == leader) {
+        kp_pq_base[w] = b0 + n;
+    }
+    if (k >= P.pq_cap)
+        return false;
+    peel_rec r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        r.x[a] = (float)L.sx[a];
+    r.ib = L.sib;
+    r.cell_o = (uint32_t)L.si[0] | ((uint32_t)L.si[1] << 8) | ((uint32_t)L.si[2] << 16) |
+               ((uint32_t)L.o << 24);
+    r.band = L.band;
+    r.A = L.Acur;
+    P.pq[k] = r;
+    return true;
+}
+
+// Marks the unused rest of the wave's queue chunk (ib = -1) when the wave
+// is done; the queue kernels skip such records.
+__device__ __forceinline__ void close_pq_chunk(const kparams &P)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
+    for (uint32_t k = kp_pq_base[w] + lane; k < e; k += 64)
+        P.pq[k].ib = -1;
+}
+
+// Returns true when the ray needs no march in this kernel (a pass-2 ray with
+// nothing to deposit, or one queued for the gradient-march kernel): the lane
+// is then pending the ray's end.
 template <class Pol, bool ADJ>
-__device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L, bool restart = false)
+__device__ __forceinline__ bool start_peel(const kparams &P, lane_t<Pol> &L, bool restart = false)
 {
     using real = typename Pol::real;
     real no[3], ev[3];
@@ -275,6 +339,16 @@ __device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L, boo
     L.tau = 0.0;
     L.kgc = -1;
     L.mode = M_PEEL;
+    if constexpr (ADJ && sizeof(typename Pol::real) == 4) {
+        // pass 2 marches a peel-off ray only for its gradient deposits
+        if (P.pq && L.pass == 2 && L.peel_sub == 1 &&
+            (L.Acur == 0.0 || !P.g_logrho || queue_peel<Pol>(P, L))) {
+            L.mode = M_PEND;
+            L.pa = PA_PEEL_END;
+            return true;
+        }
+    }
+    return false;
 }
 
 template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
@@ -403,8 +477,8 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
                 // measured an A_k that did not fit the row: march it again
                 L.Acur = peel_A<Pol>(P, L);
                 L.peel_sub = 1;
-                start_peel<Pol, ADJ>(P, L, true);
-                L.pa = PA_NONE;
+                if (!start_peel<Pol, ADJ>(P, L, true))
+                    L.pa = PA_NONE;
                 return;
             }
             if (L.ev_kind == EV_SCATTER && P.model == KP_HG_RAYLEIGH &&
@@ -463,8 +537,8 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
     }
     L.o += 1;
     if (L.o < P.n_incl) {
-        start_peel<Pol, ADJ>(P, L);
-        L.pa = PA_NONE;
+        if (!start_peel<Pol, ADJ>(P, L))
+            L.pa = PA_NONE;
         return;
     }
 #pragma unroll
@@ -566,8 +640,8 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
         save_main(L);
         L.ev_kind = EV_SCATTER;
         L.o = 0;
-        start_peel<Pol, ADJ>(P, L);
-        L.pa = PA_NONE;
+        if (!start_peel<Pol, ADJ>(P, L))
+            L.pa = PA_NONE;
     } else {
         L.pa = PA_SCATTER;
     }
@@ -609,6 +683,20 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P)
     }
 }
 
+// Peel-off rays that need no march here (queued, or nothing to deposit)
+// end at once, observer after observer, so their lanes do not wait for
+// another action phase.
+template <class Pol, bool ADJ>
+__device__ __forceinline__ void end_queued_peels(const kparams &P, lane_t<Pol> &L, bool lds)
+{
+    for (int it = 0; it <= kMaxIncl; ++it) {
+        if (!__ballot(L.mode == M_PEND && L.pa == PA_PEEL_END))
+            break;
+        if (L.mode == M_PEND && L.pa == PA_PEEL_END)
+            peel_end_stage<Pol, ADJ>(P, L, lds);
+    }
+}
+
 // lds: the launch gives each workgroup the kp_lds accumulators.
 template <class Pol, bool ADJ>
 __device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
@@ -629,6 +717,12 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
+    if constexpr (ADJ && sizeof(real) == 4) {
+        if (lane == 0) {
+            kp_pq_base[threadIdx.x >> 6] = 0;
+            kp_pq_end[threadIdx.x >> 6] = 0;
+        }
+    }
 
     for (;;) {
         const uint64_t live = __ballot(L.mode != M_DONE);
@@ -643,10 +737,16 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 peel_end_stage<Pol, ADJ>(P, L, lds);
             if (L.mode == M_PEND && L.pa == PA_INTERACT)
                 interact_stage<Pol, ADJ>(P, L);
+            if constexpr (ADJ && sizeof(real) == 4)
+                if (P.pq)
+                    end_queued_peels<Pol, ADJ>(P, L, lds);
             if (L.mode == M_PEND && L.pa == PA_SCATTER)
                 scatter_stage<Pol, ADJ>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
                 finish<Pol, ADJ>(P, L, C, L.status, lds);
+            if constexpr (ADJ && sizeof(real) == 4)
+                if (P.pq)
+                    end_queued_peels<Pol, ADJ>(P, L, lds);
             // refill idle lanes from the wave's slot range
             const uint64_t need = __ballot(L.mode == M_NEED);
             if (need) {
@@ -771,6 +871,9 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         }
     }
     jsum = L.jsum;
+    if constexpr (ADJ && sizeof(real) == 4)
+        if (P.pq)
+            close_pq_chunk(P);
 }
 
 } // namespace kp

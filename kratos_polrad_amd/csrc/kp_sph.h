@@ -365,25 +365,10 @@ __device__ __forceinline__ void deposit(const kparams &P, bool lds, bool runs, b
             atomic_add_f64(j_at(P, (uint64_t)gc, band), v);
         return;
     }
-    const int lane = threadIdx.x & 63;
-    // J element as the run key; lanes without a tally get keys of their own
-    const uint64_t key = has ? (uint64_t)gc * (uint32_t)P.j_cs + (uint64_t)band * (uint32_t)P.j_bs
-                             : ~(uint64_t)lane;
-    const uint64_t kprev = __shfl_up(key, 1, 64);
-    const uint64_t knext = __shfl_down(key, 1, 64);
-    int head = (lane == 0 || kprev != key) ? 1 : 0;
-    double sum = has ? v : 0.0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double so = __shfl_up(sum, d, 64);
-        const int ho = __shfl_up(head, d, 64);
-        if (lane >= d && !head) {
-            sum += so;
-            head = ho;
-        }
-    }
-    if (has && (lane == 63 || knext != key))
-        atomic_add_f64(j_base(P) + key, sum);
+    // the J element (j_at without the base) as the run key
+    const uint64_t e = (uint64_t)(uint32_t)gc * (uint32_t)P.j_cs +
+                       (uint64_t)(uint32_t)band * (uint32_t)P.j_bs;
+    wave_run_add(j_base(P), has, e, v);
 }
 
 template <class C>

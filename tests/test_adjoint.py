@@ -132,3 +132,29 @@ def test_gpu_f32_adjoint_tracks_f64_adjoint():
         a, b = out[_abi.KP_F32][k][sel].sum(), out[_abi.KP_F64][k][sel].sum()
         assert abs(a - b) <= 5e-3 * abs(b), (k, a, b)
     assert abs(out[_abi.KP_F32]["jlin"][0] / out[_abi.KP_F64]["jlin"][0] - 1) < 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_incl", [1, 2])
+def test_gpu_f32_peel_queue_equals_in_replay_march(n_incl):
+    """Queued pass-2 peel-off rays, marched by the separate kernel in
+    image-plane order, deposit the same gradients as the replay's own march:
+    same rays, same FP32 arithmetic, only the summation order differs."""
+    import torch
+
+    m, kappa, alb, g, run, W, sel = _setup(40000)
+    run = RunSpec(**{**run.__dict__, "precision": _abi.KP_F32,
+                     "incl_deg": (70.0, 20.0)[:n_incl]})
+    eng = Engine(m)
+    eng.set_media(kappa, alb, g)
+    w = torch.tensor(W[:, :n_incl], dtype=torch.float64, device="cuda").contiguous()
+    out = {}
+    for q in (0, 1):
+        eng.set_schedule(peel_queue=q)
+        gr = eng.adjoint(SRC, run, w)
+        torch.cuda.synchronize()
+        out[q] = {k: v.cpu().numpy() for k, v in gr.items()}
+    for k in ("d_logrho", "d_albedo", "d_g"):
+        a, b = out[1][k], out[0][k]
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
+    assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
