@@ -107,7 +107,7 @@ struct kparams {
     uint64_t adj_lanes;
     double *jlin;                  // device scalar
     peel_rec *pq;                  // queue of pass-2 peel-off rays (or null)
-    unsigned int *pq_count;
+    unsigned long long *pq_count;
     uint32_t pq_cap;
     const int32_t *pq_order;       // queue slots in image-plane order
 };

@@ -82,7 +82,7 @@ struct kp_engine {
     // mixed adjoint: queue of pass-2 peel-off rays, its sort keys, order and
     // workspace
     kp::peel_rec *pq = nullptr;
-    unsigned int *pq_count = nullptr;
+    unsigned long long *pq_count = nullptr;
     int32_t *pq_keys = nullptr, *pq_order = nullptr;
     void *pq_ws = nullptr;
     size_t pq_cap = 0, pq_ws_cap = 0;
@@ -366,14 +366,13 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
     return k_persist<kp::pol32, 4, ADJ>; // 128 VGPRs, no spills
 }
 
-__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
 
 // Sort key of every queued peel-off ray: observer, then the Morton code of
 // its start point's 2^9 x 2^9 image-plane cell, then its depth along the line
 // of sight (2^9 steps); unused slots sort last.
 __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
 {
-    const unsigned int n = min(*P.pq_count, P.pq_cap);
+    const uint64_t n = *P.pq_count < P.pq_cap ? *P.pq_count : (uint64_t)P.pq_cap;
     double D = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -422,7 +421,7 @@ __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
 // replay's own peel-off march, one thread per ray in image-plane order.
 __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
 {
-    const unsigned int n = min(*P.pq_count, P.pq_cap);
+    const uint64_t n = *P.pq_count < P.pq_cap ? *P.pq_count : (uint64_t)P.pq_cap;
     const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     double *gl = kp::g_base(P, P.g_logrho);
@@ -432,7 +431,7 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
         int64_t idx = -1;
         if (k < P.pq_cap) {
             const uint32_t s = (uint32_t)P.pq_order[k];
-            if (s < n && P.pq[s].ib >= 0)
+            if ((uint64_t)s < n && P.pq[s].ib >= 0)
                 idx = s;
         }
         // used slots sort first: a wave with none ends this thread's share
@@ -1191,12 +1190,26 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         P.batch = (uint32_t)S.event_batch;
         // mixed precision: pass-2 peel-off rays go to a queue marched in
         // image-plane order after the replay (capacity 4 rays per packet and
-        // observer; beyond it the replay marches them itself)
+        // observer, at most half of the free device memory; beyond it the
+        // replay marches them itself)
         const bool queue = run->precision == KP_F32 && S.peel_queue && grads->d_logrho &&
                            e->n_cell < 256;
         if (queue) {
-            const uint64_t want_q = std::min<uint64_t>(
+            uint64_t want_q = std::min<uint64_t>(
                 (1ull << 31) - 1, std::max<uint64_t>(1ull << 20, 4ull * total * run->n_incl));
+            if (e->pq_cap < want_q) {
+                const auto q_bytes = [](uint64_t q) {
+                    return (double)q * (sizeof(kp::peel_rec) + 2 * sizeof(int32_t)) +
+                           (double)kp_compact_workspace_bytes((int64_t)q, 1);
+                };
+                size_t free_b = 0, total_b = 0;
+                KP_HIP(hipMemGetInfo(&free_b, &total_b));
+                const double budget = 0.5 * ((double)free_b + q_bytes(e->pq_cap));
+                if (q_bytes(want_q) > budget)
+                    want_q = std::max<uint64_t>(
+                        std::max<uint64_t>(1ull << 20, e->pq_cap),
+                        (uint64_t)((double)want_q * budget / q_bytes(want_q)));
+            }
             const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
             if (e->pq_cap < want_q) {
                 dfree(e->pq);
@@ -1215,11 +1228,11 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                 e->pq_ws_cap = wsb;
             }
             if (!e->pq_count)
-                e->pq_count = dmalloc<unsigned int>(1);
+                e->pq_count = dmalloc<unsigned long long>(1);
             P.pq = e->pq;
             P.pq_count = e->pq_count;
-            P.pq_cap = (uint32_t)e->pq_cap;
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, e->pq_count);
+            P.pq_cap = (uint32_t)want_q; // this run's share of a larger cached queue
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->pq_count);
         }
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
         hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
@@ -1227,7 +1240,7 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         KP_HIP(hipGetLastError());
         if (queue) {
             hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, P, e->pq_keys);
-            kp::sort_order(e->pq_keys, e->pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
+            kp::sort_order(e->pq_keys, P.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
                            stream);
             P.pq_order = e->pq_order;
             hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0, stream,

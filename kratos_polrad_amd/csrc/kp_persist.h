@@ -229,8 +229,11 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
 // Wave-private chunk [base, end) of the peel-off queue, in LDS (one entry per
 // wave of the workgroup): a wave claims kPqChunk slots with one atomic and
 // fills them itself, so the global counter sees one atomic per chunk.
+// The counter and the bounds are 64-bit: waves keep claiming chunks past
+// the capacity (their records are then not written), so a 32-bit counter
+// could wrap back below it.
 constexpr uint32_t kPqChunk = 256;
-__shared__ uint32_t kp_pq_base[4], kp_pq_end[4];
+__shared__ uint64_t kp_pq_base[4], kp_pq_end[4];
 
 // Queue the current pass-2 peel-off ray (start point L.sx / L.si / L.sib,
 // observer L.o, A_k = L.Acur) for the gradient-march kernel.  Called by the
@@ -244,14 +247,14 @@ __device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &
     const int leader = __ffsll((long long)m) - 1;
     const uint32_t n = (uint32_t)__popcll(m);
     const uint32_t rank = (uint32_t)__popcll((lane == 0) ? 0ull : (m & (~0ull >> (64 - lane))));
-    const uint32_t b0 = kp_pq_base[w], e0 = kp_pq_end[w];
-    const uint32_t avail = e0 - b0;
-    uint64_t k = (uint64_t)b0 + rank;
+    const uint64_t b0 = kp_pq_base[w], e0 = kp_pq_end[w];
+    const uint32_t avail = (uint32_t)(e0 - b0);
+    uint64_t k = b0 + rank;
     if (n > avail) {
         // the rest of the chunk, then a new one
-        unsigned int nb = 0;
+        unsigned long long nb = 0;
         if (lane == leader)
-            nb = atomicAdd(P.pq_count, kPqChunk);
+            nb = atomicAdd(P.pq_count, (unsigned long long)kPqChunk);
         nb = __shfl(nb, leader, 64);
         if (rank >= avail)
             k = (uint64_t)nb + (rank - avail);
@@ -282,8 +285,8 @@ __device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &
 __device__ __forceinline__ void close_pq_chunk(const kparams &P)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
-    for (uint32_t k = kp_pq_base[w] + lane; k < e; k += 64)
+    const uint64_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
+    for (uint64_t k = kp_pq_base[w] + lane; k < e; k += 64)
         P.pq[k].ib = -1;
 }
 
