@@ -265,12 +265,26 @@ typedef struct kp_schedule {
      * order.  Engines on spherical meshes start with the one-thread-per-packet
      * schedule in this order. */
     int32_t coherent;
-    /* kp_adjoint, mixed precision, persistent: 1 (default) = the replay
-     * queues its peel-off rays (a 32 B record each) and a separate kernel
-     * marches them in image-plane order, lanes in lockstep, summing equal-cell
-     * gradient deposits of neighbouring lanes before one atomic; 0 = the
-     * replay marches them itself.  Same deposits, different summation order. */
+    /* kp_adjoint, mixed precision, persistent: 1 (default) = one-pass
+     * replay.  Each packet is marched once and its gradient deposits are
+     * deferred: every main-path vertex (emission or scatter) is queued as a
+     * 96 B record (the interaction's cell, albedo and scattering angle, the
+     * sums of the packet's peel-off contributions before and after its
+     * peel-offs, and the start of the next segment), every peel-off ray as a
+     * 32 B record.  Once the packets' sums are known, one kernel deposits
+     * the vertex terms and marches the segments, another marches the
+     * peel-off rays in image-plane order, lanes in lockstep, summing
+     * equal-cell deposits of neighbouring lanes before one atomic.  Packets
+     * whose records do not fit are replayed by the two-pass kernel.
+     * 0 = two-pass replay only: every packet is marched twice, the second
+     * time with its deposits.  Same deposits, different summation order. */
     int32_t peel_queue;
+    /* Queue capacity of the one-pass replay in main-path vertices per
+     * packet (0 = 32), with one peel-off ray per vertex and observer: about
+     * 3 KB + 1.3 KB per observer for each packet of a chunk.  Runs whose
+     * queues would take more than half of the free device memory are
+     * processed in chunks of packets. */
+    int32_t vertex_slots;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

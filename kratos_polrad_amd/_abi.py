@@ -107,6 +107,7 @@ class Schedule(C.Structure):
         ("waves_per_simd", C.c_int32),
         ("coherent", C.c_int32),
         ("peel_queue", C.c_int32),
+        ("vertex_slots", C.c_int32),
     ]
 
 

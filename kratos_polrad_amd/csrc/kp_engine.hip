@@ -71,7 +71,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0, 1};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0, 1, 0};
     int n_cu = 0;
     fwd_bufs fb;                        // forward launch scratch
     // adjoint: per-thread rows of peel-off contributions, gradient replicas
@@ -79,13 +79,21 @@ struct kp_engine {
     size_t adj_cap = 0;
     double *g_rep = nullptr;
     size_t g_rep_cap = 0;
-    // mixed adjoint: queue of pass-2 peel-off rays, its sort keys, order and
-    // workspace
+    // one-pass mixed adjoint: queue of peel-off rays, its sort keys, order
+    // and workspace; queue of main-path vertices; per-slot A_k sums; packets
+    // left to the two-pass kernel
     kp::peel_rec *pq = nullptr;
     unsigned long long *pq_count = nullptr;
     int32_t *pq_keys = nullptr, *pq_order = nullptr;
     void *pq_ws = nullptr;
     size_t pq_cap = 0, pq_ws_cap = 0;
+    kp::vtx_rec *vq = nullptr;
+    unsigned long long *vq_count = nullptr;
+    size_t vq_cap = 0;
+    double *pk_atot = nullptr;
+    int32_t *fb_list = nullptr;
+    unsigned int *fb_count = nullptr;
+    size_t fb_cap = 0;
     std::map<std::string, kp_schedule> tuned; // autotuner cache
 };
 
@@ -197,6 +205,8 @@ __global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // out[e] += sum_r rep[r][e]; rep[r][e] = 0 (ready for the next launch)
 __global__ void __launch_bounds__(256) k_fold(double *out, double *rep, uint64_t n, uint32_t R)
@@ -315,7 +325,7 @@ template <bool PERSIST> __global__ void __launch_bounds__(256, 4) k_sph(kp::kpar
     }
 }
 
-template <class Pol, int WPS, bool ADJ>
+template <class Pol, int WPS, bool ADJ, bool TAPE = false>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
     bool lds = false;
@@ -329,7 +339,7 @@ __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
     }
     kp::lane_counters C;
     double jsum = 0.0;
-    kp::persist_body<Pol, ADJ>(P, P.work, C, jsum, lds);
+    kp::persist_body<Pol, ADJ, TAPE>(P, P.work, C, jsum, lds);
     flush_counters(P, C);
     if constexpr (!ADJ) {
         if (P.lds_n > 0) {
@@ -364,6 +374,14 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
     if (wps == 6)
         return k_persist<kp::pol32, 6, ADJ>;
     return k_persist<kp::pol32, 4, ADJ>; // 128 VGPRs, no spills
+}
+
+// one-pass replay (deferred deposits) of mixed-precision histories
+persist_fn defer_variant(int wps)
+{
+    if (wps == 1)
+        return k_persist<kp::pol32, 1, true, true>;
+    return k_persist<kp::pol32, 4, true, true>;
 }
 
 
@@ -431,8 +449,9 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
         int64_t idx = -1;
         if (k < P.pq_cap) {
             const uint32_t s = (uint32_t)P.pq_order[k];
-            if ((uint64_t)s < n && P.pq[s].ib >= 0)
-                idx = s;
+            if ((uint64_t)s < n && P.pq[s].ib >= 0 &&
+                !(P.pk_atot && isnan(P.pk_atot[P.pq[s].slot])))
+                idx = s; // (a NaN sum marks a packet left to the two-pass kernel)
         }
         // used slots sort first: a wave with none ends this thread's share
         if (!__ballot(idx >= 0))
@@ -454,7 +473,7 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
             r.i[2] = (int)((rec.cell_o >> 16) & 255u);
             r.ib = rec.ib;
             kp::pol32::relocate(P, r);
-            ks = (float)P.bands[rec.band].kappa_scale;
+            ks = (float)P.bands[kp::band_of(P, (uint64_t)rec.slot)].kappa_scale;
             A = rec.A;
         }
         while (__ballot(live)) {
@@ -478,6 +497,105 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
                 }
             }
             kp::wave_run_add(gl, has, (uint64_t)gc, v);
+        }
+    }
+}
+
+// Deferred deposits of the one-pass mixed adjoint, one main-path vertex per
+// lane, B = Atot - C with the packet's Atot from pk_atot:
+//   interaction: d/dlog(rho) += B_v, d/d(albedo) += B_v / albedo;
+//   scatter:     d/dg += B_s dlog P_HG(mu)/dg, plus the peel-off phase terms;
+//   segment:     d/dlog(rho_c) -= kappa_c l_c B_s for every crossing of the
+//                segment that starts at the vertex, marched with the main
+//                path's own FP32 arithmetic (same cells, same lengths, same
+//                end: interaction, escape or step budget).
+// B_v / B_s: the A_k still ahead before / after the vertex's peel-offs.
+__global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
+{
+    const uint64_t n = *P.vq_count < P.vq_cap ? *P.vq_count : (uint64_t)P.vq_cap;
+    const int lane = threadIdx.x & 63;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+         base += stride) {
+        const uint64_t k = base + lane;
+        kp::vtx_rec rec;
+        bool ok = false;
+        double Atot = 0.0;
+        if (k < n) {
+            rec = P.vq[k];
+            if (rec.ib >= 0) {
+                Atot = P.pk_atot[rec.slot];
+                ok = !isnan(Atot);
+            }
+        }
+        kp::f32::ray_t r;
+        float ks = 0.0f;
+        double Bs = 0.0, tau = 0.0;
+        int steps = 0;
+        bool live = false;
+        if (ok) {
+            const int band = kp::band_of(P, (uint64_t)rec.slot);
+            const uint32_t fl = P.bands[band].flags;
+            const double Bv = Atot - rec.Cv;
+            Bs = Atot - rec.Cs;
+            if (rec.gc >= 0) {
+                if (Bv != 0.0) {
+                    if (P.g_logrho)
+                        kp::atomic_add_f64(kp::g_base(P, P.g_logrho) + rec.gc, Bv);
+                    if ((fl & KP_BAND_CELL_ALBEDO) && P.g_alb)
+                        kp::atomic_add_f64(kp::g_base(P, P.g_alb) + rec.gc, Bv / (double)rec.alb);
+                }
+                if (P.model == KP_HG_RAYLEIGH && (fl & KP_BAND_CELL_G) && P.g_g) {
+                    if (Bs != 0.0) {
+                        const double gg = P.g ? (double)P.g[rec.gc] : 0.0;
+                        kp::atomic_add_f64(kp::g_base(P, P.g_g) + rec.gc,
+                                           Bs * kp::dlogp_hg(gg, (double)rec.mu));
+                    }
+                    if (rec.phs != 0.0)
+                        kp::atomic_add_f64(kp::g_base(P, P.g_g) + rec.gc, rec.phs);
+                }
+            }
+            live = P.g_logrho && Bs != 0.0 && rec.steps > 0;
+            if (live) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    r.x[a] = rec.x[a];
+                    r.d[a] = rec.d[a];
+                }
+                r.i[0] = (int)(rec.cell & 255u);
+                r.i[1] = (int)((rec.cell >> 8) & 255u);
+                r.i[2] = (int)((rec.cell >> 16) & 255u);
+                r.ib = rec.ib;
+                kp::pol32::relocate(P, r);
+                ks = (float)P.bands[band].kappa_scale;
+                tau = rec.tau;
+                steps = rec.steps;
+            }
+        }
+        double *gl = P.g_logrho ? kp::g_base(P, P.g_logrho) : nullptr;
+        while (__ballot(live)) {
+            if (live) {
+                float l;
+                const int ap = steps > 0 ? kp::pol32::dl(r, l) : -1;
+                if (ap < 0) {
+                    live = false;
+                } else {
+                    const int64_t gc = (int64_t)r.ib * P.n3 + kp::pol32::cell(P, r);
+                    const float kap = P.kappa[gc] * ks;
+                    const float dtau = kap * l;
+                    steps -= 1;
+                    const bool hit = kap > 0.0f && (double)dtau >= tau;
+                    const float seg = hit ? fminf((float)tau * kp::f32::fast_rcp(kap), l) : l;
+                    kp::atomic_add_f64(gl + gc, -((double)(kap * seg) * Bs));
+                    if (hit) {
+                        live = false;
+                    } else {
+                        tau = tau - (double)dtau;
+                        if (kp::pol32::cross(P, r, ap, l) != kp::f64::ST_IN)
+                            live = false;
+                    }
+                }
+            }
         }
     }
 }
@@ -540,6 +658,13 @@ void free_all(kp_engine *e)
     dfree(e->pq_order);
     dfree(e->pq_ws);
     e->pq_cap = e->pq_ws_cap = 0;
+    dfree(e->vq);
+    dfree(e->vq_count);
+    e->vq_cap = 0;
+    dfree(e->pk_atot);
+    dfree(e->fb_list);
+    dfree(e->fb_count);
+    e->fb_cap = 0;
     if (e->stage)
         (void)hipHostFree(e->stage);
     e->stage = nullptr;
@@ -1188,63 +1313,142 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         P.work = e->fb.work;
         P.chunk = (uint32_t)S.chunk;
         P.batch = (uint32_t)S.event_batch;
-        // mixed precision: pass-2 peel-off rays go to a queue marched in
-        // image-plane order after the replay (capacity 4 rays per packet and
-        // observer, at most half of the free device memory; beyond it the
-        // replay marches them itself)
-        const bool queue = run->precision == KP_F32 && S.peel_queue && grads->d_logrho &&
-                           e->n_cell < 256;
-        if (queue) {
-            uint64_t want_q = std::min<uint64_t>(
-                (1ull << 31) - 1, std::max<uint64_t>(1ull << 20, 4ull * total * run->n_incl));
-            if (e->pq_cap < want_q) {
-                const auto q_bytes = [](uint64_t q) {
-                    return (double)q * (sizeof(kp::peel_rec) + 2 * sizeof(int32_t)) +
-                           (double)kp_compact_workspace_bytes((int64_t)q, 1);
-                };
+        // mixed precision, one-pass replay: each packet is marched once; its
+        // main-path vertices and peel-off rays are queued with what their
+        // deposits need and deposited by separate kernels once the packets'
+        // A_k sums are known (peel-off rays marched in image-plane order).
+        // The queues hold vertex_slots (default 32) vertices per packet and
+        // one peel-off ray per vertex and observer; runs whose queues would
+        // take more than half of the free device memory go in chunks of
+        // packets.  Packets whose records still do not fit are replayed by
+        // the two-pass kernel after the last chunk.
+        const bool any_grad = grads->d_logrho || grads->d_albedo || grads->d_g;
+        const bool defer = run->precision == KP_F32 && S.peel_queue &&
+                           (!grads->d_logrho || e->n_cell < 256) && total < (1ull << 31);
+        const bool queue = defer && grads->d_logrho;
+        const bool vqueue = defer && any_grad;
+        uint64_t chunk = total;
+        if (defer) {
+            const uint64_t vs = S.vertex_slots > 0 ? (uint64_t)S.vertex_slots : 32;
+            const uint64_t ps = vs * (uint64_t)run->n_incl;
+            const auto q_bytes = [](uint64_t q) {
+                return q ? (double)q * (sizeof(kp::peel_rec) + 2 * sizeof(int32_t)) +
+                               (double)kp_compact_workspace_bytes((int64_t)q, 1)
+                         : 0.0;
+            };
+            const auto v_bytes = [](uint64_t v) { return (double)v * sizeof(kp::vtx_rec); };
+            const auto bytes = [&](uint64_t pk) {
+                return (queue ? q_bytes(ps * pk) : 0.0) + (vqueue ? v_bytes(vs * pk) : 0.0);
+            };
+            if (vqueue) {
                 size_t free_b = 0, total_b = 0;
                 KP_HIP(hipMemGetInfo(&free_b, &total_b));
-                const double budget = 0.5 * ((double)free_b + q_bytes(e->pq_cap));
-                if (q_bytes(want_q) > budget)
-                    want_q = std::max<uint64_t>(
-                        std::max<uint64_t>(1ull << 20, e->pq_cap),
-                        (uint64_t)((double)want_q * budget / q_bytes(want_q)));
+                const double budget = 0.5 * ((double)free_b + q_bytes(e->pq_cap) +
+                                             v_bytes(e->vq_cap));
+                // largest chunk (a multiple of 2^16 packets) whose queues fit
+                while (chunk > (1ull << 16) &&
+                       (bytes(chunk) > budget || vs * chunk >= (1ull << 31) ||
+                        ps * chunk >= (1ull << 31)))
+                    chunk = std::max<uint64_t>(1ull << 16, ((chunk / 2) >> 16) << 16);
             }
-            const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
-            if (e->pq_cap < want_q) {
-                dfree(e->pq);
-                dfree(e->pq_keys);
-                dfree(e->pq_order);
-                e->pq_cap = 0;
-                e->pq = dmalloc<kp::peel_rec>(want_q);
-                e->pq_keys = dmalloc<int32_t>(want_q);
-                e->pq_order = dmalloc<int32_t>(want_q);
-                e->pq_cap = want_q;
+            const uint64_t want_q = queue ? ps * chunk : 0, want_v = vqueue ? vs * chunk : 0;
+            if (queue) {
+                const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
+                if (e->pq_cap < want_q) {
+                    dfree(e->pq);
+                    dfree(e->pq_keys);
+                    dfree(e->pq_order);
+                    e->pq_cap = 0;
+                    e->pq = dmalloc<kp::peel_rec>(want_q);
+                    e->pq_keys = dmalloc<int32_t>(want_q);
+                    e->pq_order = dmalloc<int32_t>(want_q);
+                    e->pq_cap = want_q;
+                }
+                if (e->pq_ws_cap < wsb) {
+                    dfree(e->pq_ws);
+                    e->pq_ws_cap = 0;
+                    KP_HIP(hipMalloc(&e->pq_ws, wsb));
+                    e->pq_ws_cap = wsb;
+                }
+                if (!e->pq_count)
+                    e->pq_count = dmalloc<unsigned long long>(1);
+                P.pq = e->pq;
+                P.pq_count = e->pq_count;
+                P.pq_cap = (uint32_t)want_q; // this run's share of a larger cached queue
             }
-            if (e->pq_ws_cap < wsb) {
-                dfree(e->pq_ws);
-                e->pq_ws_cap = 0;
-                KP_HIP(hipMalloc(&e->pq_ws, wsb));
-                e->pq_ws_cap = wsb;
+            if (vqueue) {
+                if (e->vq_cap < want_v) {
+                    dfree(e->vq);
+                    e->vq_cap = 0;
+                    e->vq = dmalloc<kp::vtx_rec>(want_v);
+                    e->vq_cap = want_v;
+                }
+                if (!e->vq_count)
+                    e->vq_count = dmalloc<unsigned long long>(1);
+                P.vq = e->vq;
+                P.vq_count = e->vq_count;
+                P.vq_cap = (uint32_t)want_v;
             }
-            if (!e->pq_count)
-                e->pq_count = dmalloc<unsigned long long>(1);
-            P.pq = e->pq;
-            P.pq_count = e->pq_count;
-            P.pq_cap = (uint32_t)want_q; // this run's share of a larger cached queue
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->pq_count);
+            if (e->fb_cap < total) {
+                dfree(e->fb_list);
+                dfree(e->pk_atot);
+                e->fb_cap = 0;
+                e->fb_list = dmalloc<int32_t>(total);
+                e->pk_atot = dmalloc<double>(total);
+                e->fb_cap = total;
+            }
+            if (!e->fb_count)
+                e->fb_count = dmalloc<unsigned int>(1);
+            P.pk_atot = e->pk_atot;
+            P.fb_list = e->fb_list;
+            P.fb_count = e->fb_count;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, e->fb_count);
         }
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
-        hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
-                           dim3((unsigned)blocks), dim3(T), 0, stream, P);
-        KP_HIP(hipGetLastError());
-        if (queue) {
-            hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, P, e->pq_keys);
-            kp::sort_order(e->pq_keys, P.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
-                           stream);
-            P.pq_order = e->pq_order;
-            hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0, stream,
-                               P);
+        if (!defer) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
+            hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
+                               dim3((unsigned)blocks), dim3(T), 0, stream, P);
+            KP_HIP(hipGetLastError());
+        }
+        for (uint64_t c0 = 0; defer && c0 < total; c0 += chunk) {
+            // packets [c0, c1): the claim counter starts at c0
+            kp::kparams Pc = P;
+            Pc.total = std::min<uint64_t>(total, c0 + chunk);
+            hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, stream, e->fb.work,
+                               (unsigned long long)c0);
+            if (queue)
+                hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->pq_count);
+            if (vqueue)
+                hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->vq_count);
+            hipLaunchKernelGGL(defer_variant(S.waves_per_simd), dim3((unsigned)blocks), dim3(T),
+                               0, stream, Pc);
+            KP_HIP(hipGetLastError());
+            if (queue) {
+                hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, Pc,
+                                   e->pq_keys);
+                kp::sort_order(e->pq_keys, Pc.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
+                               stream);
+                Pc.pq_order = e->pq_order;
+                hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0,
+                                   stream, Pc);
+                KP_HIP(hipGetLastError());
+            }
+            if (vqueue) {
+                hipLaunchKernelGGL(k_vtx_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0,
+                                   stream, Pc);
+                KP_HIP(hipGetLastError());
+            }
+        }
+        if (defer) {
+            // the packets left over, two-pass, in the order they were listed
+            kp::kparams F = P;
+            F.pq = nullptr;
+            F.vq = nullptr;
+            F.order = e->fb_list;
+            F.total_dev = e->fb_count;
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->fb.work);
+            hipLaunchKernelGGL(persist_variant<true>(run->precision, S.waves_per_simd),
+                               dim3((unsigned)blocks), dim3(T), 0, stream, F);
             KP_HIP(hipGetLastError());
         }
         if (P.g_nrep > 1) {
@@ -1511,6 +1715,8 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: coherent must be 0 or 1");
         if (s->peel_queue != 0 && s->peel_queue != 1)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: peel_queue must be 0 or 1");
+        if (s->vertex_slots < 0 || s->vertex_slots > 4096)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: vertex_slots must be in [0, 4096]");
         e->sched = *s;
     });
 }

@@ -19,6 +19,19 @@
 // waiting, so it executes once for many lanes instead of once per lane.
 // Each history performs exactly the per-packet arithmetic of the precision
 // policy in the same order (bit-identical to the CPU oracle in FP64).
+//
+// Adjoint launches replay each history for its gradients.  The two-pass
+// replay (ADJ) marches a packet twice: pass 1 sums its peel-off
+// contributions A_k, pass 2 re-marches it and deposits the gradients with
+// B = the A_k still ahead.  The one-pass replay (ADJ + DEFER, mixed
+// precision) marches the packet once and defers every deposit: each
+// main-path vertex becomes a record holding C = the sum of the packet's A_k
+// so far (so B = Atot - C) and the start of the segment that follows, each
+// peel-off ray a record with its A_k; the packet's Atot goes to a per-slot
+// array.  Separate kernels then deposit the vertex terms and march the
+// segments and the peel-off rays.  A packet whose records do not fit is
+// marked (Atot = NaN, its records are skipped) and left to the two-pass
+// kernel.
 #pragma once
 
 #include "kp_transport32.h"
@@ -159,6 +172,10 @@ template <class Pol> struct lane_t {
     double B;         // pass 2: sum of the A_k still ahead
     double Acur;      // A_k of the current peel-off ray
     double jsum;      // lane's sum of A_k over its packets
+    // one-pass replay (deferred deposits)
+    double Cv;        // A_k sum before the current vertex's peel-offs
+    double phs;       // the vertex's peel-off phase terms
+    int tover;        // a queue overflowed: the two-pass kernel redoes the packet
 };
 
 struct lane_counters {
@@ -172,9 +189,9 @@ __device__ __forceinline__ double dlogp_hg(double gg, double mu)
     return -2.0 * gg / (1.0 - gg * gg) - 3.0 * (gg - mu) / D;
 }
 
-template <class Pol, bool ADJ> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L);
+template <class Pol, bool ADJ, bool DEFER> __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L);
 
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_counters &C,
                                        int status, bool lds)
 {
@@ -183,11 +200,20 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
     C.cross += L.ncross;
     C.peel += L.npeel;
     C.events += (uint32_t)L.events;
-    if constexpr (ADJ) {
+    if constexpr (ADJ && DEFER) {
+        if (L.tover) {
+            P.pk_atot[L.slot] = __builtin_nan("");
+            const unsigned int f = atomicAdd(P.fb_count, 1u);
+            P.fb_list[f] = (int32_t)L.slot;
+        } else {
+            P.pk_atot[L.slot] = L.Atot;
+            L.jsum += L.Atot;
+        }
+    } else if constexpr (ADJ) {
         if (L.pass == 1) {
             // replay the same history for the gradients
             L.pass = 2;
-            emit_packet<Pol, ADJ>(P, L);
+            emit_packet<Pol, ADJ, DEFER>(P, L);
             return;
         }
         L.jsum += L.Atot;
@@ -225,46 +251,53 @@ __device__ __forceinline__ void finish(const kparams &P, lane_t<Pol> &L, lane_co
     L.mode = M_NEED;
 }
 
-// Start the peel-off ray toward observer L.o from the saved main position.
-// Wave-private chunk [base, end) of the peel-off queue, in LDS (one entry per
-// wave of the workgroup): a wave claims kPqChunk slots with one atomic and
-// fills them itself, so the global counter sees one atomic per chunk.
-// The counter and the bounds are 64-bit: waves keep claiming chunks past
-// the capacity (their records are then not written), so a 32-bit counter
-// could wrap back below it.
-constexpr uint32_t kPqChunk = 256;
-__shared__ uint64_t kp_pq_base[4], kp_pq_end[4];
+// Wave-private chunks [base, end) of the peel-off and vertex queues, in LDS
+// (one entry per wave of the workgroup): a wave claims kQChunk slots with one
+// atomic and fills them itself, so a queue's counter sees one atomic per
+// chunk, and lanes that write together write neighbouring records.  The
+// counters and bounds are 64-bit: waves keep claiming chunks past a queue's
+// capacity (those records are not written), so a 32-bit counter could wrap
+// back below it.
+constexpr uint32_t kQChunk = 256;
+__shared__ uint64_t kp_pq_base[4], kp_pq_end[4], kp_vq_base[4], kp_vq_end[4];
 
-// Queue the current pass-2 peel-off ray (start point L.sx / L.si / L.sib,
-// observer L.o, A_k = L.Acur) for the gradient-march kernel.  Called by the
-// lanes that queue, together.  Returns false beyond the queue's capacity
-// (the lane then marches the ray itself).
-template <class Pol>
-__device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &L)
+// One queue slot for each calling lane (called by those lanes together).
+__device__ __forceinline__ uint64_t claim_q(unsigned long long *count, uint64_t *qbase,
+                                            uint64_t *qend)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t m = __ballot(1);
     const int leader = __ffsll((long long)m) - 1;
     const uint32_t n = (uint32_t)__popcll(m);
     const uint32_t rank = (uint32_t)__popcll((lane == 0) ? 0ull : (m & (~0ull >> (64 - lane))));
-    const uint64_t b0 = kp_pq_base[w], e0 = kp_pq_end[w];
+    const uint64_t b0 = qbase[w], e0 = qend[w];
     const uint32_t avail = (uint32_t)(e0 - b0);
     uint64_t k = b0 + rank;
     if (n > avail) {
         // the rest of the chunk, then a new one
         unsigned long long nb = 0;
         if (lane == leader)
-            nb = atomicAdd(P.pq_count, (unsigned long long)kPqChunk);
+            nb = atomicAdd(count, (unsigned long long)kQChunk);
         nb = __shfl(nb, leader, 64);
         if (rank >= avail)
             k = (uint64_t)nb + (rank - avail);
         if (lane == leader) {
-            kp_pq_base[w] = nb + (n - avail);
-            kp_pq_end[w] = nb + kPqChunk;
+            qbase[w] = nb + (n - avail);
+            qend[w] = nb + kQChunk;
         }
     } else if (lane == leader) {
-        kp_pq_base[w] = b0 + n;
+        qbase[w] = b0 + n;
     }
+    return k;
+}
+
+// Queue the finished peel-off ray (start point L.sx / L.si / L.sib,
+// observer L.o, A_k = L.Acur) for the gradient-march kernel.  Called by the
+// lanes that queue, together.  Returns false beyond the queue's capacity.
+template <class Pol>
+__device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &L)
+{
+    const uint64_t k = claim_q(P.pq_count, kp_pq_base, kp_pq_end);
     if (k >= P.pq_cap)
         return false;
     peel_rec r;
@@ -274,27 +307,64 @@ __device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &
     r.ib = L.sib;
     r.cell_o = (uint32_t)L.si[0] | ((uint32_t)L.si[1] << 8) | ((uint32_t)L.si[2] << 16) |
                ((uint32_t)L.o << 24);
-    r.band = L.band;
+    r.slot = (int32_t)L.slot;
     r.A = L.Acur;
     P.pq[k] = r;
     return true;
 }
 
-// Marks the unused rest of the wave's queue chunk (ib = -1) when the wave
-// is done; the queue kernels skip such records.
-__device__ __forceinline__ void close_pq_chunk(const kparams &P)
+// Queue the current main-path vertex: the ray L.cur starts the next segment
+// (free flight L.tau, L.steps crossings left, or none when `ended`).  Called
+// by the lanes that queue, together.  Returns false beyond the capacity.
+template <class Pol>
+__device__ __forceinline__ bool queue_vtx(const kparams &P, const lane_t<Pol> &L, int32_t gc,
+                                          float mu, bool ended)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
-    for (uint64_t k = kp_pq_base[w] + lane; k < e; k += 64)
-        P.pq[k].ib = -1;
+    const uint64_t k = claim_q(P.vq_count, kp_vq_base, kp_vq_end);
+    if (k >= P.vq_cap)
+        return false;
+    vtx_rec r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.x[a] = (float)L.cur.x[a];
+        r.d[a] = (float)L.cur.d[a];
+    }
+    r.ib = L.cur.ib;
+    r.cell = (uint32_t)L.cur.i[0] | ((uint32_t)L.cur.i[1] << 8) | ((uint32_t)L.cur.i[2] << 16);
+    r.tau = L.tau;
+    r.Cv = L.Cv;
+    r.Cs = L.Atot;
+    r.phs = L.phs;
+    r.slot = (int32_t)L.slot;
+    r.gc = gc;
+    r.steps = ended ? 0 : L.steps;
+    r.alb = (float)L.alb;
+    r.mu = mu;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0u;
+    P.vq[k] = r;
+    return true;
 }
 
-// Returns true when the ray needs no march in this kernel (a pass-2 ray with
-// nothing to deposit, or one queued for the gradient-march kernel): the lane
-// is then pending the ray's end.
-template <class Pol, bool ADJ>
-__device__ __forceinline__ bool start_peel(const kparams &P, lane_t<Pol> &L, bool restart = false)
+// Marks the unused rest of the wave's queue chunks (ib = -1) when the wave
+// is done; the queue kernels skip such records.
+__device__ __forceinline__ void close_q_chunks(const kparams &P)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (P.pq) {
+        const uint64_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
+        for (uint64_t k = kp_pq_base[w] + lane; k < e; k += 64)
+            P.pq[k].ib = -1;
+    }
+    if (P.vq) {
+        const uint64_t e = kp_vq_end[w] < P.vq_cap ? kp_vq_end[w] : P.vq_cap;
+        for (uint64_t k = kp_vq_base[w] + lane; k < e; k += 64)
+            P.vq[k].ib = -1;
+    }
+}
+
+// Start the peel-off ray toward observer L.o from the saved main position.
+template <class Pol, bool ADJ, bool DEFER>
+__device__ __forceinline__ void start_peel(const kparams &P, lane_t<Pol> &L, bool restart = false)
 {
     using real = typename Pol::real;
     real no[3], ev[3];
@@ -318,7 +388,7 @@ __device__ __forceinline__ bool start_peel(const kparams &P, lane_t<Pol> &L, boo
             }
         }
         if constexpr (ADJ) {
-            if (L.pass == 2) {
+            if (!DEFER && L.pass == 2) {
                 const uint64_t lane_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
                 if (L.kslot < P.adj_cap) {
                     L.Acur = P.adj_rows[(uint64_t)L.kslot * P.adj_lanes + lane_id];
@@ -342,16 +412,6 @@ __device__ __forceinline__ bool start_peel(const kparams &P, lane_t<Pol> &L, boo
     L.tau = 0.0;
     L.kgc = -1;
     L.mode = M_PEEL;
-    if constexpr (ADJ && sizeof(typename Pol::real) == 4) {
-        // pass 2 marches a peel-off ray only for its gradient deposits
-        if (P.pq && L.pass == 2 && L.peel_sub == 1 &&
-            (L.Acur == 0.0 || !P.g_logrho || queue_peel<Pol>(P, L))) {
-            L.mode = M_PEND;
-            L.pa = PA_PEEL_END;
-            return true;
-        }
-    }
-    return false;
 }
 
 template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
@@ -368,7 +428,7 @@ template <class Pol> __device__ __forceinline__ void save_main(lane_t<Pol> &L)
 
 // Scatter, new free flight, Russian roulette (after the interaction's
 // peel-offs).  Leaves the lane in MAIN, or pending PA_FINISH when killed.
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
@@ -399,7 +459,7 @@ __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
         f32::cross3(L.cur.d, L.e1, e2);
         const float xi0 = f32::u01(L.u[0]);
         const float mu = (P.model == KP_THOMSON) ? f32::thomson_mu(xi0) : f32::hg_mu(L.gg, xi0);
-        if constexpr (ADJ) {
+        if constexpr (ADJ && !DEFER) {
             if (L.pass == 2 && hg && P.g_g && L.B != 0.0)
                 atomic_add_f64(g_base(P, P.g_g) + L.gc,
                                L.B * dlogp_hg((double)L.gg, (double)mu));
@@ -417,6 +477,11 @@ __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
             } else {
                 killed = true;
             }
+        }
+        if constexpr (ADJ && DEFER) {
+            if (P.vq && !L.tover)
+                if (!queue_vtx<Pol>(P, L, (int32_t)L.gc, mu, killed))
+                    L.tover = 1;
         }
     }
     L.kgc = -1;
@@ -464,11 +529,27 @@ template <class Pol> __device__ __forceinline__ double peel_A(const kparams &P, 
 // End of a peel-off ray: deposit (forward) or A_k bookkeeping (adjoint),
 // then the next observer, or resume the main path (EMIT: march on;
 // SCATTER: pending scatter).
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L, bool lds)
 {
     using real = typename Pol::real;
-    if constexpr (ADJ) {
+    if constexpr (ADJ && DEFER) {
+        const double A = peel_A<Pol>(P, L);
+        L.Atot += A;
+        if (L.ev_kind == EV_SCATTER && P.model == KP_HG_RAYLEIGH &&
+            (P.bands[L.band].flags & KP_BAND_CELL_G) && P.g_g && A != 0.0) {
+            real no[3];
+            load3(P.no[L.o], no);
+            double mo = (double)f32::dot3(L.sd, no);
+            mo = fmin(fmax(mo, -1.0), 1.0);
+            L.phs += A * dlogp_hg((double)L.gg, mo);
+        }
+        if (P.pq && P.g_logrho && A != 0.0 && !L.tover) {
+            L.Acur = A;
+            if (!queue_peel<Pol>(P, L))
+                L.tover = 1;
+        }
+    } else if constexpr (ADJ) {
         const uint64_t lane_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if (L.pass == 1) {
             const double A = peel_A<Pol>(P, L);
@@ -480,8 +561,8 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
                 // measured an A_k that did not fit the row: march it again
                 L.Acur = peel_A<Pol>(P, L);
                 L.peel_sub = 1;
-                if (!start_peel<Pol, ADJ>(P, L, true))
-                    L.pa = PA_NONE;
+                start_peel<Pol, ADJ, DEFER>(P, L, true);
+                L.pa = PA_NONE;
                 return;
             }
             if (L.ev_kind == EV_SCATTER && P.model == KP_HG_RAYLEIGH &&
@@ -540,8 +621,8 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
     }
     L.o += 1;
     if (L.o < P.n_incl) {
-        if (!start_peel<Pol, ADJ>(P, L))
-            L.pa = PA_NONE;
+        start_peel<Pol, ADJ, DEFER>(P, L);
+        L.pa = PA_NONE;
         return;
     }
 #pragma unroll
@@ -557,12 +638,20 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
     if (L.ev_kind == EV_EMIT) {
         L.mode = M_MAIN;
         L.pa = PA_NONE;
+        if constexpr (ADJ && DEFER) {
+            // the emission vertex: the first segment, no interaction terms
+            if (P.vq && !L.tover) {
+                L.Cv = L.Atot;
+                if (!queue_vtx<Pol>(P, L, -1, 0.0f, false))
+                    L.tover = 1;
+            }
+        }
     } else {
         L.pa = PA_SCATTER;
     }
 }
 
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
 {
     L.band = band_of(P, L.slot);
@@ -590,12 +679,18 @@ __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
             L.Atot = 0.0;
         else
             L.B = L.Atot;
+        if constexpr (DEFER) {
+            L.tover = 0;
+            L.Cv = 0.0;
+            L.phs = 0.0;
+            L.alb = (typename Pol::real)1;
+        }
     }
     if (P.n_incl > 0) {
         save_main(L);
         L.ev_kind = EV_EMIT;
         L.o = 0;
-        start_peel<Pol, ADJ>(P, L);
+        start_peel<Pol, ADJ, DEFER>(P, L);
     } else {
         L.mode = M_MAIN;
     }
@@ -603,7 +698,7 @@ __device__ __forceinline__ void emit_packet(const kparams &P, lane_t<Pol> &L)
 
 // Interaction at the current point: Philox words of the event, absorption,
 // then the peel-offs (or the scatter directly when there is no observer).
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
 {
     using real = typename Pol::real;
@@ -631,7 +726,10 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
     }
     L.gg = gg;
     L.alb = alb;
-    if constexpr (ADJ) {
+    if constexpr (ADJ && DEFER) {
+        L.Cv = L.Atot;
+        L.phs = 0.0;
+    } else if constexpr (ADJ) {
         if (L.pass == 2 && L.B != 0.0) {
             if (P.g_logrho)
                 atomic_add_f64(g_base(P, P.g_logrho) + gc, L.B);
@@ -643,8 +741,8 @@ __device__ __forceinline__ void interact_stage(const kparams &P, lane_t<Pol> &L)
         save_main(L);
         L.ev_kind = EV_SCATTER;
         L.o = 0;
-        if (!start_peel<Pol, ADJ>(P, L))
-            L.pa = PA_NONE;
+        start_peel<Pol, ADJ, DEFER>(P, L);
+        L.pa = PA_NONE;
     } else {
         L.pa = PA_SCATTER;
     }
@@ -686,22 +784,8 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P)
     }
 }
 
-// Peel-off rays that need no march here (queued, or nothing to deposit)
-// end at once, observer after observer, so their lanes do not wait for
-// another action phase.
-template <class Pol, bool ADJ>
-__device__ __forceinline__ void end_queued_peels(const kparams &P, lane_t<Pol> &L, bool lds)
-{
-    for (int it = 0; it <= kMaxIncl; ++it) {
-        if (!__ballot(L.mode == M_PEND && L.pa == PA_PEEL_END))
-            break;
-        if (L.mode == M_PEND && L.pa == PA_PEEL_END)
-            peel_end_stage<Pol, ADJ>(P, L, lds);
-    }
-}
-
 // lds: the launch gives each workgroup the kp_lds accumulators.
-template <class Pol, bool ADJ>
+template <class Pol, bool ADJ, bool DEFER>
 __device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
                              double &jsum, bool lds)
 {
@@ -717,13 +801,18 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     L.B = 0.0;
     L.Acur = 0.0;
     L.peel_sub = 0;
+    L.tover = 0;
+    L.Cv = 0.0;
+    L.phs = 0.0;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t wbase = 0, wend = 0; // wave-uniform slot range
-    if constexpr (ADJ && sizeof(real) == 4) {
+    if constexpr (DEFER) {
         if (lane == 0) {
             kp_pq_base[threadIdx.x >> 6] = 0;
             kp_pq_end[threadIdx.x >> 6] = 0;
+            kp_vq_base[threadIdx.x >> 6] = 0;
+            kp_vq_end[threadIdx.x >> 6] = 0;
         }
     }
 
@@ -737,19 +826,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
             // ---------------- action phase: every pending lane together
             if (L.mode == M_PEND && L.pa == PA_PEEL_END)
-                peel_end_stage<Pol, ADJ>(P, L, lds);
+                peel_end_stage<Pol, ADJ, DEFER>(P, L, lds);
             if (L.mode == M_PEND && L.pa == PA_INTERACT)
-                interact_stage<Pol, ADJ>(P, L);
-            if constexpr (ADJ && sizeof(real) == 4)
-                if (P.pq)
-                    end_queued_peels<Pol, ADJ>(P, L, lds);
+                interact_stage<Pol, ADJ, DEFER>(P, L);
             if (L.mode == M_PEND && L.pa == PA_SCATTER)
-                scatter_stage<Pol, ADJ>(P, L);
+                scatter_stage<Pol, ADJ, DEFER>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
-                finish<Pol, ADJ>(P, L, C, L.status, lds);
-            if constexpr (ADJ && sizeof(real) == 4)
-                if (P.pq)
-                    end_queued_peels<Pol, ADJ>(P, L, lds);
+                finish<Pol, ADJ, DEFER>(P, L, C, L.status, lds);
             // refill idle lanes from the wave's slot range
             const uint64_t need = __ballot(L.mode == M_NEED);
             if (need) {
@@ -772,9 +855,9 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                     wbase += n;
                 }
                 if (L.mode == M_NEED) {
-                    if (my < P.total) {
+                    if (my < P.total && (!P.total_dev || my < *P.total_dev)) {
                         L.slot = slot_at(P, my);
-                        emit_packet<Pol, ADJ>(P, L);
+                        emit_packet<Pol, ADJ, DEFER>(P, L);
                     } else {
                         L.mode = M_DONE;
                     }
@@ -828,7 +911,9 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 hit = kap > 0.0f && (double)dtau >= L.tau;
                 seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
             }
-            if constexpr (ADJ) {
+            if constexpr (ADJ && DEFER) {
+                // deposited by the vertex kernel
+            } else if constexpr (ADJ) {
                 if (L.pass == 2 && P.g_logrho && L.B != 0.0)
                     atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)(kap * seg) * L.B));
             } else if constexpr (sizeof(real) == 8) {
@@ -848,7 +933,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         } else {
             L.tau = L.tau + (double)dtau;
             L.npeel += 1;
-            if constexpr (ADJ) {
+            if constexpr (ADJ && !DEFER) {
                 if (L.pass == 2 && L.peel_sub == 1 && P.g_logrho && L.Acur != 0.0)
                     atomic_add_f64(g_base(P, P.g_logrho) + gc, -((double)dtau * L.Acur));
             }
@@ -874,9 +959,8 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         }
     }
     jsum = L.jsum;
-    if constexpr (ADJ && sizeof(real) == 4)
-        if (P.pq)
-            close_pq_chunk(P);
+    if constexpr (DEFER)
+        close_q_chunks(P);
 }
 
 } // namespace kp

@@ -136,10 +136,17 @@ def test_gpu_f32_adjoint_tracks_f64_adjoint():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n_incl", [1, 2])
-def test_gpu_f32_peel_queue_equals_in_replay_march(n_incl):
-    """Queued pass-2 peel-off rays, marched by the separate kernel in
-    image-plane order, deposit the same gradients as the replay's own march:
-    same rays, same FP32 arithmetic, only the summation order differs."""
+@pytest.mark.parametrize("vertex_slots,wps", [(0, 0), (3, 0), (1, 1)],
+                         ids=["default", "vq3_fallback", "vq1_fallback_wps1"])
+def test_gpu_f32_one_pass_replay_equals_two_pass(n_incl, vertex_slots, wps):
+    """The one-pass replay (deposits deferred: main-path vertices and
+    peel-off rays queued, deposited by separate kernels once each packet's
+    sum of peel-off contributions is known) deposits the same gradients as
+    the two-pass replay: same histories, same FP32 arithmetic per deposit,
+    B = Atot - C instead of a running difference, different summation
+    order.  Short vertex queues (1 or 3 records per packet) send packets to
+    the two-pass kernel, which must neither lose nor double-count them (their
+    queued records are skipped)."""
     import torch
 
     m, kappa, alb, g, run, W, sel = _setup(40000)
@@ -150,11 +157,12 @@ def test_gpu_f32_peel_queue_equals_in_replay_march(n_incl):
     w = torch.tensor(W[:, :n_incl], dtype=torch.float64, device="cuda").contiguous()
     out = {}
     for q in (0, 1):
-        eng.set_schedule(peel_queue=q)
+        eng.set_schedule(peel_queue=q, vertex_slots=vertex_slots, waves_per_simd=wps)
         gr = eng.adjoint(SRC, run, w)
         torch.cuda.synchronize()
         out[q] = {k: v.cpu().numpy() for k, v in gr.items()}
     for k in ("d_logrho", "d_albedo", "d_g"):
         a, b = out[1][k], out[0][k]
+        assert np.any(b != 0.0), k
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
     assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
