@@ -236,12 +236,24 @@ __device__ __forceinline__ uint32_t bin_key(const int32_t *keys, uint64_t i, int
     return (k >= 0 && k < n_bins) ? (uint32_t)k : (uint32_t)n_bins;
 }
 
+// n_dev (optional): a device-side count; only the first min(n, *n_dev) keys
+// are sorted (the order past them is not written).
+__device__ __forceinline__ uint64_t n_eff(uint64_t n, const unsigned long long *n_dev)
+{
+    if (!n_dev)
+        return n;
+    const uint64_t d = *n_dev;
+    return d < n ? d : n;
+}
+
 template <bool FIRST>
 __global__ void __launch_bounds__(kThreads) k_radix_hist(const int32_t *keys0,
                                                        const uint32_t *keys, uint64_t n,
                                                        int32_t n_bins, int shift,
-                                                       uint32_t *hist, uint64_t n_tiles)
+                                                       uint32_t *hist, uint64_t n_tiles,
+                                                       const unsigned long long *n_dev)
 {
+    n = n_eff(n, n_dev);
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -264,8 +276,11 @@ template <bool FIRST>
 __global__ void __launch_bounds__(kThreads) k_radix_scatter(
     const int32_t *keys0, const uint32_t *keys_in, const uint32_t *idx_in, uint32_t *keys_out,
     uint32_t *idx_out, uint64_t n, int32_t n_bins, int shift, const uint32_t *hist_off,
-    uint64_t n_tiles)
+    uint64_t n_tiles, const unsigned long long *n_dev)
 {
+    n = n_eff(n, n_dev);
+    if ((uint64_t)blockIdx.x * kTile >= n)
+        return;
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[kThreads / 64][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -434,8 +449,10 @@ template <class F> void by_word(int32_t unit, const void *a, const void *b, F &&
 }
 
 __global__ void __launch_bounds__(kThreads) k_copy_order(const uint32_t *sidx, uint64_t n,
-                                                       int32_t *order)
+                                                       int32_t *order,
+                                                       const unsigned long long *n_dev)
 {
+    n = n_eff(n, n_dev);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
         order[p] = (int32_t)sidx[p];
@@ -444,7 +461,7 @@ __global__ void __launch_bounds__(kThreads) k_copy_order(const uint32_t *sidx, u
 } // namespace
 
 void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void *ws,
-                    size_t ws_bytes, hipStream_t s)
+                    size_t ws_bytes, hipStream_t s, const unsigned long long *n_dev)
 {
     if (n >= (1ull << 31) || bits < 1 || bits > 30)
         throw kp::error(KP_ERR_ARG, "sort_order: n must be below 2^31 and bits in [1,30]");
@@ -466,24 +483,25 @@ void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, v
         const int shift = 8 * p;
         if (p == 0)
             hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nt), dim3(kThreads), 0, s, keys,
-                               (const uint32_t *)nullptr, n, lim, shift, hist, nt);
+                               (const uint32_t *)nullptr, n, lim, shift, hist, nt, n_dev);
         else
             hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
-                               keys, kin, n, lim, shift, hist, nt);
+                               keys, kin, n, lim, shift, hist, nt, n_dev);
         exscan(hist, 256 * nt, sw, nullptr, s);
         if (p == 0)
             hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nt), dim3(kThreads), 0, s,
                                keys, (const uint32_t *)nullptr, (const uint32_t *)nullptr, kout,
-                               iout, n, lim, shift, hist, nt);
+                               iout, n, lim, shift, hist, nt, n_dev);
         else
             hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
-                               keys, kin, iin, kout, iout, n, lim, shift, hist, nt);
+                               keys, kin, iin, kout, iout, n, lim, shift, hist, nt, n_dev);
         kin = kout;
         iin = iout;
         kout = (kout == ka) ? kb : ka;
         iout = (iout == ia) ? ib : ia;
     }
-    hipLaunchKernelGGL(k_copy_order, dim3(grid_for(n)), dim3(kThreads), 0, s, iin, n, order);
+    hipLaunchKernelGGL(k_copy_order, dim3(grid_for(n)), dim3(kThreads), 0, s, iin, n, order,
+                       n_dev);
     KP_HIPC(hipGetLastError());
 }
 
@@ -586,20 +604,21 @@ kp_status kp_bin_by_key(const int32_t *keys, int64_t n, int32_t n_bins, int64_t 
             if (p == 0)
                 hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nt), dim3(kThreads), 0,
                                    s, keys, (const uint32_t *)nullptr, (uint64_t)n, n_bins,
-                                   shift, hist, nt);
+                                   shift, hist, nt, (const unsigned long long *)nullptr);
             else
                 hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nt), dim3(kThreads), 0,
-                                   s, keys, kin, (uint64_t)n, n_bins, shift, hist, nt);
+                                   s, keys, kin, (uint64_t)n, n_bins, shift, hist, nt,
+                                   (const unsigned long long *)nullptr);
             exscan(hist, 256 * nt, sw, nullptr, s);
             if (p == 0)
                 hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nt), dim3(kThreads), 0,
                                    s, keys, (const uint32_t *)nullptr,
                                    (const uint32_t *)nullptr, kout, iout, (uint64_t)n, n_bins,
-                                   shift, hist, nt);
+                                   shift, hist, nt, (const unsigned long long *)nullptr);
             else
                 hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nt), dim3(kThreads),
                                    0, s, keys, kin, iin, kout, iout, (uint64_t)n, n_bins,
-                                   shift, hist, nt);
+                                   shift, hist, nt, (const unsigned long long *)nullptr);
             kin = kout;
             iin = iout;
             kout = (kout == ka) ? kb : ka;

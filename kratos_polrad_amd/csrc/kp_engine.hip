@@ -397,11 +397,7 @@ __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
         D += (P.xmax[a] - P.xmin[a]) * (P.xmax[a] - P.xmin[a]);
     D = 0.5 * sqrt(D);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.pq_cap; k += stride) {
-        if (k >= n) {
-            keys[k] = (int32_t)((1u << 30) - 1u);
-            continue;
-        }
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         const kp::peel_rec rec = P.pq[k];
         if (rec.ib < 0) { // unused rest of a wave's chunk
             keys[k] = (int32_t)((1u << 30) - 1u);
@@ -443,18 +439,22 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
     const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     double *gl = kp::g_base(P, P.g_logrho);
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-         base < P.pq_cap; base += stride) {
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+         base += stride) {
         const uint64_t k = base + lane;
         int64_t idx = -1;
-        if (k < P.pq_cap) {
+        bool used = false;
+        if (k < n) {
             const uint32_t s = (uint32_t)P.pq_order[k];
-            if ((uint64_t)s < n && P.pq[s].ib >= 0 &&
-                !(P.pk_atot && isnan(P.pk_atot[P.pq[s].slot])))
-                idx = s; // (a NaN sum marks a packet left to the two-pass kernel)
+            if ((uint64_t)s < n && P.pq[s].ib >= 0) {
+                used = true;
+                // a NaN sum marks a packet left to the two-pass kernel
+                if (!(P.pk_atot && isnan(P.pk_atot[P.pq[s].slot])))
+                    idx = s;
+            }
         }
         // used slots sort first: a wave with none ends this thread's share
-        if (!__ballot(idx >= 0))
+        if (!__ballot(used))
             break;
         kp::f32::ray_t r;
         float ks = 0.0f;
@@ -1427,7 +1427,7 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                 hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, Pc,
                                    e->pq_keys);
                 kp::sort_order(e->pq_keys, Pc.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
-                               stream);
+                               stream, e->pq_count);
                 Pc.pq_order = e->pq_order;
                 hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0,
                                    stream, Pc);

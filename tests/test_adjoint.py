@@ -166,3 +166,29 @@ def test_gpu_f32_one_pass_replay_equals_two_pass(n_incl, vertex_slots, wps):
         assert np.any(b != 0.0), k
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
     assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_f32_one_pass_replay_many_records_with_fallback():
+    """More queued peel-off rays than threads of the gradient kernels (each
+    thread takes several), with many packets left to the two-pass kernel (2
+    vertex records per packet): the records of those packets are interleaved
+    with valid ones in image-plane order and must be skipped without ending a
+    thread's share early."""
+    import torch
+
+    m, kappa, alb, g, run, W, sel = _setup(300000)
+    run = RunSpec(**{**run.__dict__, "precision": _abi.KP_F32})
+    eng = Engine(m)
+    eng.set_media(kappa, alb, g)
+    w = torch.tensor(W, dtype=torch.float64, device="cuda")
+    out = {}
+    for q in (0, 1):
+        eng.set_schedule(peel_queue=q, vertex_slots=2)
+        gr = eng.adjoint(SRC, run, w)
+        torch.cuda.synchronize()
+        out[q] = {k: v.cpu().numpy() for k, v in gr.items()}
+    for k in ("d_logrho", "d_albedo", "d_g"):
+        a, b = out[1][k], out[0][k]
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
+    assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
