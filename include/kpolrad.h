@@ -258,6 +258,11 @@ typedef struct kp_schedule {
      * the emission draws), radix-sorted before the transport kernel, so the
      * lanes of a wave fly neighbouring rays: their steps stay in lockstep and
      * equal-cell tallies of neighbouring lanes are summed before one atomic.
+     * With the mixed-precision persistent kind, a wave takes new packets
+     * only once all its lanes are idle (lanes that start together stay in
+     * step) and main-path J deposits are summed over runs of lanes without
+     * the per-lane FP32 accumulator: faster on optically thin bands, slower
+     * on thick ones (kp_autotune measures both).
      * Histories are unchanged (end_state stays indexed by slot); tallies
      * differ by summation order only.  Takes about 26 B of scratch per
      * packet; launches of 2^31 packets or more, or of more than 2^22 bands,

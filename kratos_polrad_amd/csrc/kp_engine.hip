@@ -325,7 +325,7 @@ template <bool PERSIST> __global__ void __launch_bounds__(256, 4) k_sph(kp::kpar
     }
 }
 
-template <class Pol, int WPS, bool ADJ, bool TAPE = false>
+template <class Pol, int WPS, bool ADJ, bool DEFER = false, bool SYNC = false>
 __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
 {
     bool lds = false;
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256, WPS) k_persist(kp::kparams P)
     }
     kp::lane_counters C;
     double jsum = 0.0;
-    kp::persist_body<Pol, ADJ, TAPE>(P, P.work, C, jsum, lds);
+    kp::persist_body<Pol, ADJ, DEFER, SYNC>(P, P.work, C, jsum, lds);
     flush_counters(P, C);
     if constexpr (!ADJ) {
         if (P.lds_n > 0) {
@@ -375,6 +375,10 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
         return k_persist<kp::pol32, 6, ADJ>;
     return k_persist<kp::pol32, 4, ADJ>; // 128 VGPRs, no spills
 }
+
+// mixed forward in emission-direction order: waves refilled together, J
+// deposits summed over runs of lanes
+persist_fn sync_variant() { return k_persist<kp::pol32, 4, false, false, true>; }
 
 // one-pass replay (deferred deposits) of mixed-precision histories
 persist_fn defer_variant(int wps)
@@ -1026,9 +1030,12 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
                 hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)pl.blocks), dim3(pl.T), 0,
                                    stream, P);
         } else {
+            // mixed precision in emission-direction order: the lockstep variant
+            const persist_fn f = (run->precision == KP_F32 && P.order)
+                                     ? sync_variant()
+                                     : persist_variant<false>(run->precision, S.waves_per_simd);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.work);
-            hipLaunchKernelGGL(persist_variant<false>(run->precision, S.waves_per_simd),
-                               dim3((unsigned)pl.blocks), dim3(pl.T),
+            hipLaunchKernelGGL(f, dim3((unsigned)pl.blocks), dim3(pl.T),
                                (size_t)P.lds_n * sizeof(double), stream, P);
         }
     }
@@ -1646,15 +1653,19 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
             };
             const bool sph_mesh = e->coord == KP_COORD_SPHERICAL;
             // schedule kind and packet order together: lockstep histories pay
-            // off in emission-direction order; persistent lanes refill at
-            // different times, lose the lockstep and then only collide on
-            // the same J addresses, so that pair is not a candidate.  With
+            // off in emission-direction order.  FP64 persistent lanes refill
+            // at different times, lose the lockstep and then only collide on
+            // the same J addresses, so that pair is not a candidate; the
+            // mixed persistent kernel in that order refills whole waves and
+            // sums runs of lanes (wins on optically thin bands).  With
             // observers the one-thread-per-packet kernels march every
             // peel-off ray inside the history, unbatched: not a candidate.
             for (int kind : {KP_SCHED_PERSISTENT, KP_SCHED_HISTORY})
                 for (int coh : {0, 1}) {
+                    const bool lockstep_persist = run->precision == KP_F32 && !sph_mesh;
                     if ((cur.kind == kind && cur.coherent == coh) ||
-                        (coh && (total >= (1ull << 31) || kind == KP_SCHED_PERSISTENT)) ||
+                        (coh && (total >= (1ull << 31) ||
+                                 (kind == KP_SCHED_PERSISTENT && !lockstep_persist))) ||
                         (kind == KP_SCHED_HISTORY && run->n_incl > 0))
                         continue;
                     kp_schedule c = cur;

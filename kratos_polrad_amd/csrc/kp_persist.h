@@ -785,7 +785,12 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P)
 }
 
 // lds: the launch gives each workgroup the kp_lds accumulators.
-template <class Pol, bool ADJ, bool DEFER>
+// SYNC (mixed forward in emission-direction order): a wave takes new packets
+// only when all its lanes are idle, so lanes that start together fly
+// neighbouring rays in step; each main-path J deposit is handed to the next
+// loop iteration, where the wave sums runs of lanes in one cell before one
+// atomic (no per-lane register accumulator).
+template <class Pol, bool ADJ, bool DEFER, bool SYNC = false>
 __device__ void persist_body(const kparams &P, unsigned long long *work, lane_counters &C,
                              double &jsum, bool lds)
 {
@@ -816,7 +821,22 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         }
     }
 
+    bool jh = false; // SYNC: a J deposit waits for the wave (element jidx, value jv)
+    uint64_t jidx = 0;
+    double jv = 0.0;
+
     for (;;) {
+        if constexpr (SYNC) {
+            if (__ballot(jh)) {
+                const uint64_t key = jh ? jidx : ~(uint64_t)lane;
+                const uint64_t kprev = __shfl_up(key, 1, 64);
+                if (__ballot(jh && lane > 0 && kprev == key))
+                    wave_run_add(j_base(P), jh, jidx, jv);
+                else if (jh)
+                    atomic_add_f64(j_base(P) + jidx, jv);
+            }
+            jh = false;
+        }
         const uint64_t live = __ballot(L.mode != M_DONE);
         if (!live)
             break;
@@ -833,9 +853,10 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 scatter_stage<Pol, ADJ, DEFER>(P, L);
             if (L.mode == M_PEND && L.pa == PA_FINISH)
                 finish<Pol, ADJ, DEFER>(P, L, C, L.status, lds);
-            // refill idle lanes from the wave's slot range
+            // refill idle lanes from the wave's slot range (SYNC: only once
+            // every lane is idle)
             const uint64_t need = __ballot(L.mode == M_NEED);
-            if (need) {
+            if (need && (!SYNC || !__ballot(L.mode != M_NEED && L.mode != M_DONE))) {
                 const uint32_t n = __popcll(need);
                 const uint32_t rank = __popcll(need & lt_mask);
                 const uint64_t avail = wend - wbase;
@@ -919,6 +940,18 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             } else if constexpr (sizeof(real) == 8) {
                 if (P.J)
                     j_deposit(P, lds, gc, L.band, L.I * (seg / P.vol0[L.cur.ib]));
+            } else if constexpr (SYNC) {
+                if (P.J) {
+                    const double v = (double)(L.I * seg * L.cur.iv);
+                    if (lds && gc == P.src_gc) {
+                        atomicAdd(&kp_lds[L.band], v);
+                    } else {
+                        jh = true;
+                        jidx = (uint64_t)(uint32_t)gc * (uint32_t)P.j_cs +
+                               (uint64_t)(uint32_t)L.band * (uint32_t)P.j_bs;
+                        jv = v;
+                    }
+                }
             } else {
                 L.acc.add(P, L.band, gc, L.I * seg * L.cur.iv, lds);
             }

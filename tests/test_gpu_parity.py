@@ -142,6 +142,29 @@ def test_f32_persistent_equals_history():
     np.testing.assert_allclose(a["J"], b["J"], rtol=2e-6, atol=1e-9 * np.abs(b["J"]).max())
 
 
+@pytest.mark.parametrize("bt", [256, 128])
+def test_f32_lockstep_coherent_equals_persistent(bt):
+    """The mixed persistent schedule in emission-direction order (waves
+    refilled together, J deposits of a run of lanes in one cell summed before
+    one atomic, no per-lane accumulator) runs the same FP32 histories: end
+    states and counters identical, image identical up to summation order, J up
+    to the accumulator's FP32 rounding.  Two bands, so runs must not merge
+    different bands' elements."""
+    w = _amr_workload(6000, precision=_abi.KP_F32)
+    run = RunSpec(**{**w.run.__dict__, "bands": [
+        BandSpec(n_packets=6000, cell_albedo=True, cell_g=True),
+        BandSpec(n_packets=4000, kappa_scale=0.2, cell_albedo=True, cell_g=True)]})
+    w = w.__class__(**{**w.__dict__, "run": run})
+    a = _run_gpu(w, sched=SCHEDULES[0])
+    b = _run_gpu(w, sched=dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=bt, blocks_per_cu=4,
+                               chunk=64, coherent=1))
+    np.testing.assert_array_equal(a["end_state"], b["end_state"])
+    assert np.array_equal(a["counters"], b["counters"])
+    np.testing.assert_allclose(b["image"], a["image"], rtol=1e-11,
+                               atol=1e-12 * np.abs(a["image"]).max())
+    np.testing.assert_allclose(b["J"], a["J"], rtol=2e-6, atol=1e-9 * np.abs(a["J"]).max())
+
+
 @pytest.mark.parametrize("chunk,bt,batch", [(1, 64, 1), (16, 256, 16), (1000, 128, 64)])
 def test_persistent_refill_processes_each_packet_once(chunk, bt, batch):
     """Wave slot ranges never overlap and batched action phases keep every
