@@ -290,6 +290,9 @@ typedef struct kp_schedule {
      * queues would take more than half of the free device memory are
      * processed in chunks of packets. */
     int32_t vertex_slots;
+    /* Largest chunk of the one-pass replay, in packets (0 = as large as the
+     * queue memory allows; otherwise at least 1024). */
+    int32_t adjoint_chunk;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

@@ -108,6 +108,7 @@ class Schedule(C.Structure):
         ("coherent", C.c_int32),
         ("peel_queue", C.c_int32),
         ("vertex_slots", C.c_int32),
+        ("adjoint_chunk", C.c_int32),
     ]
 
 

@@ -71,7 +71,7 @@ struct kp_engine {
     int bands_cap = 0;
     std::vector<unsigned char> shadow;
     // launch shape
-    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0, 1, 0};
+    kp_schedule sched{KP_SCHED_PERSISTENT, 256, 4, 64, 0, 0, 8, 0, 0, 1, 0, 0};
     int n_cu = 0;
     fwd_bufs fb;                        // forward launch scratch
     // adjoint: per-thread rows of peel-off contributions, gradient replicas
@@ -1358,6 +1358,8 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                         ps * chunk >= (1ull << 31)))
                     chunk = std::max<uint64_t>(1ull << 16, ((chunk / 2) >> 16) << 16);
             }
+            if (S.adjoint_chunk > 0)
+                chunk = std::min<uint64_t>(chunk, (uint64_t)S.adjoint_chunk);
             const uint64_t want_q = queue ? ps * chunk : 0, want_v = vqueue ? vs * chunk : 0;
             if (queue) {
                 const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
@@ -1728,6 +1730,8 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: peel_queue must be 0 or 1");
         if (s->vertex_slots < 0 || s->vertex_slots > 4096)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: vertex_slots must be in [0, 4096]");
+        if (s->adjoint_chunk != 0 && s->adjoint_chunk < 1024)
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: adjoint_chunk must be 0 or >= 1024");
         e->sched = *s;
     });
 }
