@@ -514,6 +514,8 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
 //                path's own FP32 arithmetic (same cells, same lengths, same
 //                end: interaction, escape or step budget).
 // B_v / B_s: the A_k still ahead before / after the vertex's peel-offs.
+// The interaction's d/dlog(rho) term and the segment's first crossing (same
+// cell) go out as one atomic.
 __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
 {
     const uint64_t n = *P.vq_count < P.vq_cap ? *P.vq_count : (uint64_t)P.vq_cap;
@@ -537,6 +539,8 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
         double Bs = 0.0, tau = 0.0;
         int steps = 0;
         bool live = false;
+        double carry = 0.0; // interaction term waiting for the first crossing
+        int64_t carry_gc = -1;
         if (ok) {
             const int band = kp::band_of(P, (uint64_t)rec.slot);
             const uint32_t fl = P.bands[band].flags;
@@ -544,19 +548,22 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
             Bs = Atot - rec.Cs;
             if (rec.gc >= 0) {
                 if (Bv != 0.0) {
-                    if (P.g_logrho)
-                        kp::atomic_add_f64(kp::g_base(P, P.g_logrho) + rec.gc, Bv);
+                    if (P.g_logrho) {
+                        carry = Bv;
+                        carry_gc = rec.gc;
+                    }
                     if ((fl & KP_BAND_CELL_ALBEDO) && P.g_alb)
                         kp::atomic_add_f64(kp::g_base(P, P.g_alb) + rec.gc, Bv / (double)rec.alb);
                 }
                 if (P.model == KP_HG_RAYLEIGH && (fl & KP_BAND_CELL_G) && P.g_g) {
+                    // scatter term and peel-off phase terms: one cell, one atomic
+                    double v = rec.phs;
                     if (Bs != 0.0) {
                         const double gg = P.g ? (double)P.g[rec.gc] : 0.0;
-                        kp::atomic_add_f64(kp::g_base(P, P.g_g) + rec.gc,
-                                           Bs * kp::dlogp_hg(gg, (double)rec.mu));
+                        v += Bs * kp::dlogp_hg(gg, (double)rec.mu);
                     }
-                    if (rec.phs != 0.0)
-                        kp::atomic_add_f64(kp::g_base(P, P.g_g) + rec.gc, rec.phs);
+                    if (v != 0.0)
+                        kp::atomic_add_f64(kp::g_base(P, P.g_g) + rec.gc, v);
                 }
             }
             live = P.g_logrho && Bs != 0.0 && rec.steps > 0;
@@ -590,7 +597,12 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
                     steps -= 1;
                     const bool hit = kap > 0.0f && (double)dtau >= tau;
                     const float seg = hit ? fminf((float)tau * kp::f32::fast_rcp(kap), l) : l;
-                    kp::atomic_add_f64(gl + gc, -((double)(kap * seg) * Bs));
+                    double v = -((double)(kap * seg) * Bs);
+                    if (gc == carry_gc) {
+                        v += carry;
+                        carry_gc = -1;
+                    }
+                    kp::atomic_add_f64(gl + gc, v);
                     if (hit) {
                         live = false;
                     } else {
@@ -601,6 +613,8 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
                 }
             }
         }
+        if (carry_gc >= 0)
+            kp::atomic_add_f64(kp::g_base(P, P.g_logrho) + carry_gc, carry);
     }
 }
 
