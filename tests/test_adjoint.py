@@ -218,3 +218,38 @@ def test_gpu_f32_one_pass_replay_in_chunks(vertex_slots):
         a, b = out[1][k], out[0][k]
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
     assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [_abi.KP_HG_RAYLEIGH, _abi.KP_THOMSON], ids=["hg", "thomson"])
+def test_gpu_f32_one_pass_replay_several_bands(model):
+    """Three bands with their own extinction scale, albedo and g (per-cell
+    or per-band) and uneven packet counts: the deferred kernels find each
+    record's band from its packet slot.  Same gradients as the two-pass
+    replay, for HG-Rayleigh and for Thomson scattering (no g terms)."""
+    import torch
+
+    m, kappa, alb, g, run, W, sel = _setup(30000)
+    bands = [BandSpec(n_packets=30000, cell_albedo=True, cell_g=True),
+             BandSpec(n_packets=7000, kappa_scale=0.3, albedo=0.7, g=0.4),
+             BandSpec(n_packets=15000, kappa_scale=2.0, cell_albedo=True, g=0.1)]
+    run = RunSpec(**{**run.__dict__, "bands": bands, "precision": _abi.KP_F32,
+                     "scatter_model": model})
+    eng = Engine(m)
+    eng.set_media(kappa, alb, g)
+    rng = np.random.default_rng(7)
+    w = torch.tensor(rng.normal(size=(3, 2, 16, 16, 3)), dtype=torch.float64, device="cuda")
+    out = {}
+    for q in (0, 1):
+        eng.set_schedule(peel_queue=q)
+        gr = eng.adjoint(SRC, run, w)
+        torch.cuda.synchronize()
+        out[q] = {k: v.cpu().numpy() for k, v in gr.items()}
+    for k in ("d_logrho", "d_albedo", "d_g"):
+        a, b = out[1][k], out[0][k]
+        if model == _abi.KP_THOMSON and k == "d_g":
+            assert not np.any(a) and not np.any(b)
+            continue
+        assert np.any(b != 0.0), k
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
+    assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
