@@ -20,15 +20,16 @@ struct dband {
     uint32_t pad;
 };
 
-// A main-path vertex of a one-pass mixed adjoint replay (emission or
-// scatter), with what its deferred gradient deposits need once the packet's
-// sum of peel-off contributions Atot is known: B = Atot - C, C being the sum
-// of the packet's A_k before the point.  Written when the vertex's scatter is
-// done; the segment that follows starts here.
-struct vtx_rec {
-    float x[3];      // segment start: block-local position, block, cell
+// A main-path vertex of a one-pass adjoint replay (emission or scatter),
+// with what its deferred gradient deposits need once the packet's sum of
+// peel-off contributions Atot is known: B = Atot - C, C being the sum of the
+// packet's A_k before the point.  Written when the vertex's scatter is done;
+// the segment that follows starts here.  R is the transport precision
+// (FP32: block-local position; FP64: global position).
+template <class R> struct vtx_rec_t {
+    R x[3];          // segment start: position, block, cell
     int32_t ib;
-    float d[3];      // direction of the segment
+    R d[3];          // direction of the segment
     uint32_t cell;   // i0 | i1 << 8 | i2 << 16
     double tau;      // free-flight optical depth of the segment
     double Cv;       // A_k sum before the vertex's peel-offs (interaction terms)
@@ -37,16 +38,15 @@ struct vtx_rec {
     int32_t slot;    // packet slot
     int32_t gc;      // interaction cell, -1 for the emission vertex
     int32_t steps;   // main-path crossings left (0: the packet ended here)
-    float alb;       // albedo at the interaction
-    float mu;        // cosine of the scattering angle
-    uint32_t pad[3];
+    R alb;           // albedo at the interaction
+    R mu;            // cosine of the scattering angle
 };
 
 // A peel-off ray of an adjoint replay whose gradient deposits are marched
-// by a separate kernel: start point (block-local FP32), block, cell, observer,
-// packet slot and its A_k.
-struct peel_rec {
-    float x[3];
+// by a separate kernel: start point (position as in vtx_rec_t), block, cell,
+// observer, packet slot and its A_k.
+template <class R> struct peel_rec_t {
+    R x[3];
     int32_t ib;
     uint32_t cell_o; // i0 | i1 << 8 | i2 << 16 | observer << 24
     int32_t slot;
@@ -128,14 +128,13 @@ struct kparams {
     uint32_t adj_cap;
     uint64_t adj_lanes;
     double *jlin;                  // device scalar
-    peel_rec *pq;                  // queue of peel-off rays (or null)
+    void *pq;                      // queue of peel_rec_t of the precision (or null)
     unsigned long long *pq_count;
     uint32_t pq_cap;
     const int32_t *pq_order;       // queue slots in image-plane order
-    vtx_rec *vq;                   // main-path vertices of the one-pass replay
+    void *vq;                      // vtx_rec_t of the one-pass replay (or null)
     unsigned long long *vq_count;
     uint32_t vq_cap;
-    const int32_t *vq_order;       // vertex records in segment-start order
     double *pk_atot;               // per slot: Atot, or NaN for a packet left over
     int32_t *fb_list;              // slots of the packets left to the two-pass kernel
     unsigned int *fb_count;

@@ -79,17 +79,18 @@ struct kp_engine {
     size_t adj_cap = 0;
     double *g_rep = nullptr;
     size_t g_rep_cap = 0;
-    // one-pass mixed adjoint: queue of peel-off rays, its sort keys, order
-    // and workspace; queue of main-path vertices; per-slot A_k sums; packets
-    // left to the two-pass kernel
-    kp::peel_rec *pq = nullptr;
+    // one-pass adjoint: queue of peel-off rays (records of the run's
+    // precision), its sort keys, order and workspace; queue of main-path
+    // vertices; per-slot A_k sums; packets left to the two-pass kernel.
+    // Queue capacities in bytes (keys and order in records).
+    void *pq = nullptr;
     unsigned long long *pq_count = nullptr;
     int32_t *pq_keys = nullptr, *pq_order = nullptr;
     void *pq_ws = nullptr;
-    size_t pq_cap = 0, pq_ws_cap = 0;
-    kp::vtx_rec *vq = nullptr;
+    size_t pq_bytes = 0, pq_kcap = 0, pq_ws_cap = 0;
+    void *vq = nullptr;
     unsigned long long *vq_count = nullptr;
-    size_t vq_cap = 0;
+    size_t vq_bytes = 0;
     double *pk_atot = nullptr;
     int32_t *fb_list = nullptr;
     unsigned int *fb_count = nullptr;
@@ -380,9 +381,11 @@ template <bool ADJ> persist_fn persist_variant(int precision, int wps)
 // deposits summed over runs of lanes
 persist_fn sync_variant() { return k_persist<kp::pol32, 4, false, false, true>; }
 
-// one-pass replay (deferred deposits) of mixed-precision histories
-persist_fn defer_variant(int wps)
+// one-pass replay (deferred deposits)
+persist_fn defer_variant(int precision, int wps)
 {
+    if (precision == KP_F64)
+        return k_persist<kp::pol64, 1, true, true>;
     if (wps == 1)
         return k_persist<kp::pol32, 1, true, true>;
     return k_persist<kp::pol32, 4, true, true>;
@@ -392,8 +395,11 @@ persist_fn defer_variant(int wps)
 // Sort key of every queued peel-off ray: observer, then the Morton code of
 // its start point's 2^9 x 2^9 image-plane cell, then its depth along the line
 // of sight (2^9 steps); unused slots sort last.
+template <class Pol>
 __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
 {
+    using real = typename Pol::real;
+    const kp::peel_rec_t<real> *pq = static_cast<const kp::peel_rec_t<real> *>(P.pq);
     const uint64_t n = *P.pq_count < P.pq_cap ? *P.pq_count : (uint64_t)P.pq_cap;
     double D = 0.0;
 #pragma unroll
@@ -402,19 +408,19 @@ __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
     D = 0.5 * sqrt(D);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const kp::peel_rec rec = P.pq[k];
+        const kp::peel_rec_t<real> rec = pq[k];
         if (rec.ib < 0) { // unused rest of a wave's chunk
             keys[k] = (int32_t)((1u << 30) - 1u);
             continue;
         }
         const int o = (int)(rec.cell_o >> 24);
-        kp::f32::ray_t t;
+        typename Pol::ray t;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             t.x[a] = rec.x[a];
         t.ib = rec.ib;
-        float g[3];
-        kp::f32::global_pos(P, t, g);
+        real g[3];
+        Pol::gpos(P, t, g);
         double uh = 0.0, uv = 0.0, dz = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -437,8 +443,10 @@ __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
 // Gradient deposits of the queued peel-off rays, d/dlog(rho_c) += -dtau_c A_k
 // for every crossing: the same FP32 march, optical depth and cut-off as the
 // replay's own peel-off march, one thread per ray in image-plane order.
-__global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
+template <class Pol> __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
 {
+    using real = typename Pol::real;
+    const kp::peel_rec_t<real> *pq = static_cast<const kp::peel_rec_t<real> *>(P.pq);
     const uint64_t n = *P.pq_count < P.pq_cap ? *P.pq_count : (uint64_t)P.pq_cap;
     const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -450,34 +458,34 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
         bool used = false;
         if (k < n) {
             const uint32_t s = (uint32_t)P.pq_order[k];
-            if ((uint64_t)s < n && P.pq[s].ib >= 0) {
+            if ((uint64_t)s < n && pq[s].ib >= 0) {
                 used = true;
                 // a NaN sum marks a packet left to the two-pass kernel
-                if (!(P.pk_atot && isnan(P.pk_atot[P.pq[s].slot])))
+                if (!(P.pk_atot && isnan(P.pk_atot[pq[s].slot])))
                     idx = s;
             }
         }
         // used slots sort first: a wave with none ends this thread's share
         if (!__ballot(used))
             break;
-        kp::f32::ray_t r;
-        float ks = 0.0f;
+        typename Pol::ray r;
+        real ks = 0;
         double A = 0.0, tau = 0.0;
         bool live = idx >= 0;
         if (live) {
-            const kp::peel_rec rec = P.pq[idx];
+            const kp::peel_rec_t<real> rec = pq[idx];
             const int o = (int)(rec.cell_o >> 24);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 r.x[a] = rec.x[a];
-                r.d[a] = (float)P.no[o][a];
+                r.d[a] = (real)P.no[o][a];
             }
             r.i[0] = (int)(rec.cell_o & 255u);
             r.i[1] = (int)((rec.cell_o >> 8) & 255u);
             r.i[2] = (int)((rec.cell_o >> 16) & 255u);
             r.ib = rec.ib;
-            kp::pol32::relocate(P, r);
-            ks = (float)P.bands[kp::band_of(P, (uint64_t)rec.slot)].kappa_scale;
+            Pol::relocate(P, r);
+            ks = (real)P.bands[kp::band_of(P, (uint64_t)rec.slot)].kappa_scale;
             A = rec.A;
         }
         while (__ballot(live)) {
@@ -485,18 +493,18 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
             int64_t gc = 0;
             double v = 0.0;
             if (live) {
-                float l;
-                const int ap = kp::pol32::dl(r, l);
+                real l;
+                const int ap = Pol::dl(r, l);
                 if (ap < 0) {
                     live = false;
                 } else {
-                    gc = (int64_t)r.ib * P.n3 + kp::pol32::cell(P, r);
-                    const float kap = P.kappa[gc] * ks;
-                    const float dtau = kap * l;
+                    gc = (int64_t)r.ib * P.n3 + Pol::cell(P, r);
+                    const real kap = (real)P.kappa[gc] * ks;
+                    const real dtau = kap * l;
                     tau = tau + (double)dtau;
                     has = true;
                     v = -((double)dtau * A);
-                    if (tau > kp::kTauCut || kp::pol32::cross(P, r, ap, l) != kp::f64::ST_IN)
+                    if (tau > kp::kTauCut || Pol::cross(P, r, ap, l) != kp::f64::ST_IN)
                         live = false;
                 }
             }
@@ -516,26 +524,28 @@ __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
 // B_v / B_s: the A_k still ahead before / after the vertex's peel-offs.
 // The interaction's d/dlog(rho) term and the segment's first crossing (same
 // cell) go out as one atomic.
-__global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
+template <class Pol> __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
 {
+    using real = typename Pol::real;
+    const kp::vtx_rec_t<real> *vq = static_cast<const kp::vtx_rec_t<real> *>(P.vq);
     const uint64_t n = *P.vq_count < P.vq_cap ? *P.vq_count : (uint64_t)P.vq_cap;
     const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
          base += stride) {
         const uint64_t k = base + lane;
-        kp::vtx_rec rec;
+        kp::vtx_rec_t<real> rec;
         bool ok = false;
         double Atot = 0.0;
         if (k < n) {
-            rec = P.vq[k];
+            rec = vq[k];
             if (rec.ib >= 0) {
                 Atot = P.pk_atot[rec.slot];
                 ok = !isnan(Atot);
             }
         }
-        kp::f32::ray_t r;
-        float ks = 0.0f;
+        typename Pol::ray r;
+        real ks = 0;
         double Bs = 0.0, tau = 0.0;
         int steps = 0;
         bool live = false;
@@ -577,8 +587,8 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
                 r.i[1] = (int)((rec.cell >> 8) & 255u);
                 r.i[2] = (int)((rec.cell >> 16) & 255u);
                 r.ib = rec.ib;
-                kp::pol32::relocate(P, r);
-                ks = (float)P.bands[band].kappa_scale;
+                Pol::relocate(P, r);
+                ks = (real)P.bands[band].kappa_scale;
                 tau = rec.tau;
                 steps = rec.steps;
             }
@@ -586,17 +596,29 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
         double *gl = P.g_logrho ? kp::g_base(P, P.g_logrho) : nullptr;
         while (__ballot(live)) {
             if (live) {
-                float l;
-                const int ap = steps > 0 ? kp::pol32::dl(r, l) : -1;
+                real l;
+                const int ap = steps > 0 ? Pol::dl(r, l) : -1;
                 if (ap < 0) {
                     live = false;
                 } else {
-                    const int64_t gc = (int64_t)r.ib * P.n3 + kp::pol32::cell(P, r);
-                    const float kap = P.kappa[gc] * ks;
-                    const float dtau = kap * l;
+                    const int64_t gc = (int64_t)r.ib * P.n3 + Pol::cell(P, r);
+                    const real kap = (real)P.kappa[gc] * ks;
+                    const real dtau = kap * l;
                     steps -= 1;
-                    const bool hit = kap > 0.0f && (double)dtau >= tau;
-                    const float seg = hit ? fminf((float)tau * kp::f32::fast_rcp(kap), l) : l;
+                    // the main path's own segment arithmetic of the precision
+                    bool hit;
+                    real seg;
+                    if constexpr (sizeof(real) == 8) {
+                        hit = kap > 0.0 && dtau >= tau;
+                        seg = l;
+                        if (hit) {
+                            const double li = tau / kap;
+                            seg = li > l ? l : li;
+                        }
+                    } else {
+                        hit = kap > 0.0f && (double)dtau >= tau;
+                        seg = hit ? fminf((float)tau * kp::f32::fast_rcp(kap), l) : l;
+                    }
                     double v = -((double)(kap * seg) * Bs);
                     if (gc == carry_gc) {
                         v += carry;
@@ -607,7 +629,7 @@ __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
                         live = false;
                     } else {
                         tau = tau - (double)dtau;
-                        if (kp::pol32::cross(P, r, ap, l) != kp::f64::ST_IN)
+                        if (Pol::cross(P, r, ap, l) != kp::f64::ST_IN)
                             live = false;
                     }
                 }
@@ -675,10 +697,10 @@ void free_all(kp_engine *e)
     dfree(e->pq_keys);
     dfree(e->pq_order);
     dfree(e->pq_ws);
-    e->pq_cap = e->pq_ws_cap = 0;
+    e->pq_bytes = e->pq_kcap = e->pq_ws_cap = 0;
     dfree(e->vq);
     dfree(e->vq_count);
-    e->vq_cap = 0;
+    e->vq_bytes = 0;
     dfree(e->pk_atot);
     dfree(e->fb_list);
     dfree(e->fb_count);
@@ -1334,8 +1356,8 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         P.work = e->fb.work;
         P.chunk = (uint32_t)S.chunk;
         P.batch = (uint32_t)S.event_batch;
-        // mixed precision, one-pass replay: each packet is marched once; its
-        // main-path vertices and peel-off rays are queued with what their
+        // one-pass replay (either precision): each packet is marched once;
+        // its main-path vertices and peel-off rays are queued with what their
         // deposits need and deposited by separate kernels once the packets'
         // A_k sums are known (peel-off rays marched in image-plane order).
         // The queues hold vertex_slots (default 32) vertices per packet and
@@ -1344,28 +1366,36 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         // packets.  Packets whose records still do not fit are replayed by
         // the two-pass kernel after the last chunk.
         const bool any_grad = grads->d_logrho || grads->d_albedo || grads->d_g;
-        const bool defer = run->precision == KP_F32 && S.peel_queue &&
-                           (!grads->d_logrho || e->n_cell < 256) && total < (1ull << 31);
+        const bool defer = S.peel_queue && (!grads->d_logrho || e->n_cell < 256) &&
+                           total < (1ull << 31);
+        const bool f64 = run->precision == KP_F64;
+        const size_t prec_bytes = f64 ? sizeof(kp::peel_rec_t<double>)
+                                      : sizeof(kp::peel_rec_t<float>);
+        const size_t vrec_bytes = f64 ? sizeof(kp::vtx_rec_t<double>)
+                                      : sizeof(kp::vtx_rec_t<float>);
         const bool queue = defer && grads->d_logrho;
         const bool vqueue = defer && any_grad;
         uint64_t chunk = total;
         if (defer) {
             const uint64_t vs = S.vertex_slots > 0 ? (uint64_t)S.vertex_slots : 32;
             const uint64_t ps = vs * (uint64_t)run->n_incl;
-            const auto q_bytes = [](uint64_t q) {
-                return q ? (double)q * (sizeof(kp::peel_rec) + 2 * sizeof(int32_t)) +
+            const auto q_bytes = [&](uint64_t q) {
+                return q ? (double)q * (prec_bytes + 2 * sizeof(int32_t)) +
                                (double)kp_compact_workspace_bytes((int64_t)q, 1)
                          : 0.0;
             };
-            const auto v_bytes = [](uint64_t v) { return (double)v * sizeof(kp::vtx_rec); };
+            const auto v_bytes = [&](uint64_t v) { return (double)v * vrec_bytes; };
             const auto bytes = [&](uint64_t pk) {
                 return (queue ? q_bytes(ps * pk) : 0.0) + (vqueue ? v_bytes(vs * pk) : 0.0);
             };
             if (vqueue) {
                 size_t free_b = 0, total_b = 0;
                 KP_HIP(hipMemGetInfo(&free_b, &total_b));
-                const double budget = 0.5 * ((double)free_b + q_bytes(e->pq_cap) +
-                                             v_bytes(e->vq_cap));
+                // memory already held by the queues counts as free
+                const double budget =
+                    0.5 * ((double)free_b + (double)e->pq_bytes +
+                           (double)e->pq_kcap * 2 * sizeof(int32_t) + (double)e->pq_ws_cap +
+                           (double)e->vq_bytes);
                 // largest chunk (a multiple of 2^16 packets) whose queues fit
                 while (chunk > (1ull << 16) &&
                        (bytes(chunk) > budget || vs * chunk >= (1ull << 31) ||
@@ -1377,15 +1407,19 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
             const uint64_t want_q = queue ? ps * chunk : 0, want_v = vqueue ? vs * chunk : 0;
             if (queue) {
                 const size_t wsb = kp_compact_workspace_bytes((int64_t)want_q, 1);
-                if (e->pq_cap < want_q) {
+                if (e->pq_bytes < want_q * prec_bytes) {
                     dfree(e->pq);
+                    e->pq_bytes = 0;
+                    e->pq = dmalloc<char>(want_q * prec_bytes);
+                    e->pq_bytes = want_q * prec_bytes;
+                }
+                if (e->pq_kcap < want_q) {
                     dfree(e->pq_keys);
                     dfree(e->pq_order);
-                    e->pq_cap = 0;
-                    e->pq = dmalloc<kp::peel_rec>(want_q);
+                    e->pq_kcap = 0;
                     e->pq_keys = dmalloc<int32_t>(want_q);
                     e->pq_order = dmalloc<int32_t>(want_q);
-                    e->pq_cap = want_q;
+                    e->pq_kcap = want_q;
                 }
                 if (e->pq_ws_cap < wsb) {
                     dfree(e->pq_ws);
@@ -1400,11 +1434,11 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                 P.pq_cap = (uint32_t)want_q; // this run's share of a larger cached queue
             }
             if (vqueue) {
-                if (e->vq_cap < want_v) {
+                if (e->vq_bytes < want_v * vrec_bytes) {
                     dfree(e->vq);
-                    e->vq_cap = 0;
-                    e->vq = dmalloc<kp::vtx_rec>(want_v);
-                    e->vq_cap = want_v;
+                    e->vq_bytes = 0;
+                    e->vq = dmalloc<char>(want_v * vrec_bytes);
+                    e->vq_bytes = want_v * vrec_bytes;
                 }
                 if (!e->vq_count)
                     e->vq_count = dmalloc<unsigned long long>(1);
@@ -1443,22 +1477,31 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                 hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->pq_count);
             if (vqueue)
                 hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, e->vq_count);
-            hipLaunchKernelGGL(defer_variant(S.waves_per_simd), dim3((unsigned)blocks), dim3(T),
-                               0, stream, Pc);
+            hipLaunchKernelGGL(defer_variant(run->precision, S.waves_per_simd),
+                               dim3((unsigned)blocks), dim3(T), 0, stream, Pc);
             KP_HIP(hipGetLastError());
+            const dim3 gq((unsigned)(e->n_cu * 8)), tq(256);
             if (queue) {
-                hipLaunchKernelGGL(k_peel_keys, dim3(2048), dim3(256), 0, stream, Pc,
-                                   e->pq_keys);
+                if (f64)
+                    hipLaunchKernelGGL(k_peel_keys<kp::pol64>, dim3(2048), tq, 0, stream, Pc,
+                                       e->pq_keys);
+                else
+                    hipLaunchKernelGGL(k_peel_keys<kp::pol32>, dim3(2048), tq, 0, stream, Pc,
+                                       e->pq_keys);
                 kp::sort_order(e->pq_keys, Pc.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
                                stream, e->pq_count);
                 Pc.pq_order = e->pq_order;
-                hipLaunchKernelGGL(k_peel_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0,
-                                   stream, Pc);
+                if (f64)
+                    hipLaunchKernelGGL(k_peel_grad<kp::pol64>, gq, tq, 0, stream, Pc);
+                else
+                    hipLaunchKernelGGL(k_peel_grad<kp::pol32>, gq, tq, 0, stream, Pc);
                 KP_HIP(hipGetLastError());
             }
             if (vqueue) {
-                hipLaunchKernelGGL(k_vtx_grad, dim3((unsigned)(e->n_cu * 8)), dim3(256), 0,
-                                   stream, Pc);
+                if (f64)
+                    hipLaunchKernelGGL(k_vtx_grad<kp::pol64>, gq, tq, 0, stream, Pc);
+                else
+                    hipLaunchKernelGGL(k_vtx_grad<kp::pol32>, gq, tq, 0, stream, Pc);
                 KP_HIP(hipGetLastError());
             }
         }

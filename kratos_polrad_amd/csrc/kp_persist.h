@@ -297,19 +297,20 @@ __device__ __forceinline__ uint64_t claim_q(unsigned long long *count, uint64_t 
 template <class Pol>
 __device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &L)
 {
+    using real = typename Pol::real;
     const uint64_t k = claim_q(P.pq_count, kp_pq_base, kp_pq_end);
     if (k >= P.pq_cap)
         return false;
-    peel_rec r;
+    peel_rec_t<real> r;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        r.x[a] = (float)L.sx[a];
+        r.x[a] = L.sx[a];
     r.ib = L.sib;
     r.cell_o = (uint32_t)L.si[0] | ((uint32_t)L.si[1] << 8) | ((uint32_t)L.si[2] << 16) |
                ((uint32_t)L.o << 24);
     r.slot = (int32_t)L.slot;
     r.A = L.Acur;
-    P.pq[k] = r;
+    static_cast<peel_rec_t<real> *>(P.pq)[k] = r;
     return true;
 }
 
@@ -318,16 +319,17 @@ __device__ __forceinline__ bool queue_peel(const kparams &P, const lane_t<Pol> &
 // by the lanes that queue, together.  Returns false beyond the capacity.
 template <class Pol>
 __device__ __forceinline__ bool queue_vtx(const kparams &P, const lane_t<Pol> &L, int32_t gc,
-                                          float mu, bool ended)
+                                          typename Pol::real mu, bool ended)
 {
+    using real = typename Pol::real;
     const uint64_t k = claim_q(P.vq_count, kp_vq_base, kp_vq_end);
     if (k >= P.vq_cap)
         return false;
-    vtx_rec r;
+    vtx_rec_t<real> r;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        r.x[a] = (float)L.cur.x[a];
-        r.d[a] = (float)L.cur.d[a];
+        r.x[a] = L.cur.x[a];
+        r.d[a] = L.cur.d[a];
     }
     r.ib = L.cur.ib;
     r.cell = (uint32_t)L.cur.i[0] | ((uint32_t)L.cur.i[1] << 8) | ((uint32_t)L.cur.i[2] << 16);
@@ -338,27 +340,26 @@ __device__ __forceinline__ bool queue_vtx(const kparams &P, const lane_t<Pol> &L
     r.slot = (int32_t)L.slot;
     r.gc = gc;
     r.steps = ended ? 0 : L.steps;
-    r.alb = (float)L.alb;
+    r.alb = L.alb;
     r.mu = mu;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0u;
-    P.vq[k] = r;
+    static_cast<vtx_rec_t<real> *>(P.vq)[k] = r;
     return true;
 }
 
 // Marks the unused rest of the wave's queue chunks (ib = -1) when the wave
 // is done; the queue kernels skip such records.
-__device__ __forceinline__ void close_q_chunks(const kparams &P)
+template <class R> __device__ __forceinline__ void close_q_chunks(const kparams &P)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (P.pq) {
         const uint64_t e = kp_pq_end[w] < P.pq_cap ? kp_pq_end[w] : P.pq_cap;
         for (uint64_t k = kp_pq_base[w] + lane; k < e; k += 64)
-            P.pq[k].ib = -1;
+            static_cast<peel_rec_t<R> *>(P.pq)[k].ib = -1;
     }
     if (P.vq) {
         const uint64_t e = kp_vq_end[w] < P.vq_cap ? kp_vq_end[w] : P.vq_cap;
         for (uint64_t k = kp_vq_base[w] + lane; k < e; k += 64)
-            P.vq[k].ib = -1;
+            static_cast<vtx_rec_t<R> *>(P.vq)[k].ib = -1;
     }
 }
 
@@ -439,7 +440,7 @@ __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
         f64::cross3(L.cur.d, L.e1, e2);
         const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(L.u[0]))
                                                   : kp_hg_mu(L.gg, kp_u01(L.u[0]));
-        if constexpr (ADJ) {
+        if constexpr (ADJ && !DEFER) {
             if (L.pass == 2 && hg && P.g_g && L.B != 0.0)
                 atomic_add_f64(g_base(P, P.g_g) + L.gc, L.B * dlogp_hg(L.gg, mu));
         }
@@ -454,6 +455,11 @@ __device__ __forceinline__ void scatter_stage(const kparams &P, lane_t<Pol> &L)
             } else {
                 killed = true;
             }
+        }
+        if constexpr (ADJ && DEFER) {
+            if (P.vq && !L.tover)
+                if (!queue_vtx<Pol>(P, L, (int32_t)L.gc, mu, killed))
+                    L.tover = 1;
         }
     } else {
         f32::cross3(L.cur.d, L.e1, e2);
@@ -540,7 +546,11 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
             (P.bands[L.band].flags & KP_BAND_CELL_G) && P.g_g && A != 0.0) {
             real no[3];
             load3(P.no[L.o], no);
-            double mo = (double)f32::dot3(L.sd, no);
+            double mo;
+            if constexpr (sizeof(real) == 8)
+                mo = f64::dot3(L.sd, no);
+            else
+                mo = (double)f32::dot3(L.sd, no);
             mo = fmin(fmax(mo, -1.0), 1.0);
             L.phs += A * dlogp_hg((double)L.gg, mo);
         }
@@ -993,7 +1003,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
     }
     jsum = L.jsum;
     if constexpr (DEFER)
-        close_q_chunks(P);
+        close_q_chunks<real>(P);
 }
 
 } // namespace kp

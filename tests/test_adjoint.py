@@ -69,14 +69,16 @@ def test_adjoint_matches_crn_finite_differences(oracle_lib, param):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", [_abi.KP_SCHED_PERSISTENT, _abi.KP_SCHED_HISTORY],
-                         ids=["persistent", "history"])
-def test_gpu_adjoint_matches_cpu_adjoint(oracle_lib, kind):
+@pytest.mark.parametrize("kind,one_pass", [(_abi.KP_SCHED_PERSISTENT, 1),
+                                           (_abi.KP_SCHED_PERSISTENT, 0),
+                                           (_abi.KP_SCHED_HISTORY, 0)],
+                         ids=["persistent_one_pass", "persistent_two_pass", "history"])
+def test_gpu_adjoint_matches_cpu_adjoint(oracle_lib, kind, one_pass):
     import torch
 
     m, kappa, alb, g, run, W, sel = _setup(20000)
     eng = Engine(m)
-    eng.set_schedule(kind=kind)
+    eng.set_schedule(kind=kind, peel_queue=one_pass)
     eng.set_media(kappa, alb, g)
     run64 = RunSpec(**{**run.__dict__, "precision": _abi.KP_F64})
     w = torch.tensor(W, dtype=torch.float64, device="cuda")
@@ -135,22 +137,23 @@ def test_gpu_f32_adjoint_tracks_f64_adjoint():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", [_abi.KP_F32, _abi.KP_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("n_incl", [1, 2])
 @pytest.mark.parametrize("vertex_slots,wps", [(0, 0), (3, 0), (1, 1)],
                          ids=["default", "vq3_fallback", "vq1_fallback_wps1"])
-def test_gpu_f32_one_pass_replay_equals_two_pass(n_incl, vertex_slots, wps):
+def test_gpu_one_pass_replay_equals_two_pass(prec, n_incl, vertex_slots, wps):
     """The one-pass replay (deposits deferred: main-path vertices and
     peel-off rays queued, deposited by separate kernels once each packet's
     sum of peel-off contributions is known) deposits the same gradients as
-    the two-pass replay: same histories, same FP32 arithmetic per deposit,
-    B = Atot - C instead of a running difference, different summation
-    order.  Short vertex queues (1 or 3 records per packet) send packets to
+    the two-pass replay: same histories, same arithmetic per deposit in
+    either precision, B = Atot - C instead of a running difference,
+    different summation order.  Short vertex queues (1 or 3 records per packet) send packets to
     the two-pass kernel, which must neither lose nor double-count them (their
     queued records are skipped)."""
     import torch
 
     m, kappa, alb, g, run, W, sel = _setup(40000)
-    run = RunSpec(**{**run.__dict__, "precision": _abi.KP_F32,
+    run = RunSpec(**{**run.__dict__, "precision": prec,
                      "incl_deg": (70.0, 20.0)[:n_incl]})
     eng = Engine(m)
     eng.set_media(kappa, alb, g)
@@ -195,8 +198,9 @@ def test_gpu_f32_one_pass_replay_many_records_with_fallback():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", [_abi.KP_F32, _abi.KP_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("vertex_slots", [0, 2])
-def test_gpu_f32_one_pass_replay_in_chunks(vertex_slots):
+def test_gpu_one_pass_replay_in_chunks(prec, vertex_slots):
     """The one-pass replay in chunks of packets (the claim counter starting at
     each chunk's first index, the queues reused per chunk, left-over packets
     of every chunk replayed two-pass at the end) gives the same gradients as
@@ -204,7 +208,7 @@ def test_gpu_f32_one_pass_replay_in_chunks(vertex_slots):
     import torch
 
     m, kappa, alb, g, run, W, sel = _setup(40000)
-    run = RunSpec(**{**run.__dict__, "precision": _abi.KP_F32})
+    run = RunSpec(**{**run.__dict__, "precision": prec})
     eng = Engine(m)
     eng.set_media(kappa, alb, g)
     w = torch.tensor(W, dtype=torch.float64, device="cuda")
