@@ -270,13 +270,13 @@ typedef struct kp_schedule {
      * order.  Engines on spherical meshes start with the one-thread-per-packet
      * schedule in this order. */
     int32_t coherent;
-    /* kp_adjoint, mixed precision, persistent: 1 (default) = one-pass
-     * replay.  Each packet is marched once and its gradient deposits are
+    /* kp_adjoint, persistent schedule, either precision: 1 (default) =
+     * one-pass replay.  Each packet is marched once and its gradient deposits are
      * deferred: every main-path vertex (emission or scatter) is queued as a
-     * 96 B record (the interaction's cell, albedo and scattering angle, the
-     * sums of the packet's peel-off contributions before and after its
-     * peel-offs, and the start of the next segment), every peel-off ray as a
-     * 32 B record.  Once the packets' sums are known, one kernel deposits
+     * record (88 B in FP32, 128 B in FP64: the interaction's cell, albedo
+     * and scattering angle, the sums of the packet's peel-off contributions
+     * before and after its peel-offs, and the start of the next segment),
+     * every peel-off ray as a 32 B (FP32) or 48 B (FP64) record.  Once the packets' sums are known, one kernel deposits
      * the vertex terms and marches the segments, another marches the
      * peel-off rays in image-plane order, lanes in lockstep, summing
      * equal-cell deposits of neighbouring lanes before one atomic.  Packets
@@ -286,7 +286,8 @@ typedef struct kp_schedule {
     int32_t peel_queue;
     /* Queue capacity of the one-pass replay in main-path vertices per
      * packet (0 = 32), with one peel-off ray per vertex and observer: about
-     * 3 KB + 1.3 KB per observer for each packet of a chunk.  Runs whose
+     * 2.8 KB + 1.3 KB per observer for each packet of a chunk in FP32
+     * (4 KB + 1.8 KB in FP64).  Runs whose
      * queues would take more than half of the free device memory are
      * processed in chunks of packets. */
     int32_t vertex_slots;
