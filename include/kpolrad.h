@@ -287,9 +287,9 @@ typedef struct kp_schedule {
     /* Queue capacity of the one-pass replay in main-path vertices per
      * packet (0 = 32), with one peel-off ray per vertex and observer: about
      * 2.8 KB + 1.3 KB per observer for each packet of a chunk in FP32
-     * (4 KB + 1.8 KB in FP64).  Runs whose
-     * queues would take more than half of the free device memory are
-     * processed in chunks of packets. */
+     * (4 KB + 1.8 KB in FP64).  Runs whose queues would take more than half
+     * of the free device memory, or 48 GB, are processed in chunks of
+     * packets.  The engine keeps the queues until it is destroyed. */
     int32_t vertex_slots;
     /* Largest chunk of the one-pass replay, in packets (0 = as large as the
      * queue memory allows; otherwise at least 1024). */

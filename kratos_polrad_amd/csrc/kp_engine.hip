@@ -930,6 +930,7 @@ void fill_params(kp_engine *e, const kp_source *src, const kp_run *run, hipStrea
 // buffers (never called inside a stream capture).
 // bytes of LDS accumulators a persistent forward workgroup may take
 constexpr uint64_t kMaxLdsAcc = 32768;
+constexpr double kMaxQueueBytes = 48e9; // one-pass adjoint queues, per engine
 
 struct fwd_plan {
     uint64_t blocks = 0;
@@ -1362,8 +1363,8 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
         // A_k sums are known (peel-off rays marched in image-plane order).
         // The queues hold vertex_slots (default 32) vertices per packet and
         // one peel-off ray per vertex and observer; runs whose queues would
-        // take more than half of the free device memory go in chunks of
-        // packets.  Packets whose records still do not fit are replayed by
+        // take more than half of the free device memory, or 48 GB, go in
+        // chunks of packets.  Packets whose records still do not fit are replayed by
         // the two-pass kernel after the last chunk.
         const bool any_grad = grads->d_logrho || grads->d_albedo || grads->d_g;
         const bool defer = S.peel_queue && (!grads->d_logrho || e->n_cell < 256) &&
@@ -1391,16 +1392,27 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
             if (vqueue) {
                 size_t free_b = 0, total_b = 0;
                 KP_HIP(hipMemGetInfo(&free_b, &total_b));
-                // memory already held by the queues counts as free
-                const double budget =
+                // memory already held by the queues counts as free; the
+                // engine keeps the queues until it is destroyed, so at most
+                // kMaxQueueBytes
+                const double budget = std::min(
+                    kMaxQueueBytes,
                     0.5 * ((double)free_b + (double)e->pq_bytes +
                            (double)e->pq_kcap * 2 * sizeof(int32_t) + (double)e->pq_ws_cap +
-                           (double)e->vq_bytes);
-                // largest chunk (a multiple of 2^16 packets) whose queues fit
-                while (chunk > (1ull << 16) &&
-                       (bytes(chunk) > budget || vs * chunk >= (1ull << 31) ||
-                        ps * chunk >= (1ull << 31)))
-                    chunk = std::max<uint64_t>(1ull << 16, ((chunk / 2) >> 16) << 16);
+                           (double)e->vq_bytes));
+                // the largest chunk whose queues fit (at least 2^16 packets),
+                // then equal chunks of the run
+                const auto fits = [&](uint64_t c) {
+                    return bytes(c) <= budget && vs * c < (1ull << 31) && ps * c < (1ull << 31);
+                };
+                if (!fits(chunk)) {
+                    const double per = bytes(1ull << 20) / (double)(1ull << 20);
+                    uint64_t cmax = std::max<uint64_t>(1ull << 16, (uint64_t)(budget / per));
+                    while (cmax > (1ull << 16) && !fits(cmax))
+                        cmax = std::max<uint64_t>(1ull << 16, cmax - cmax / 16);
+                    const uint64_t nch = (total + cmax - 1) / cmax;
+                    chunk = (total + nch - 1) / nch;
+                }
             }
             if (S.adjoint_chunk > 0)
                 chunk = std::min<uint64_t>(chunk, (uint64_t)S.adjoint_chunk);
