@@ -43,7 +43,7 @@ typedef enum kp_status {
     KP_ERR_HIP = 2,         /* HIP runtime error (reference hip_api throws)   */
     KP_ERR_ALLOC = 3,       /* allocation failure                             */
     KP_ERR_STATE = 4,       /* call order violated (e.g. no media set)        */
-    KP_ERR_UNSUPPORTED = 5  /* valid request this build does not implement    */
+    KP_ERR_UNSUPPORTED = 5  /* valid request the engine does not implement   */
 } kp_status;
 
 /* Message of the last failure on this thread ("" if none). */

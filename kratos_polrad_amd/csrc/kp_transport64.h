@@ -2,7 +2,8 @@
 //
 // The plain method: the reference march of photon.h:20-125 / proxy.h:64-121
 // in FP64 global coordinates (the PRECISION=2 build of types.h:12-27), with
-// the north-star physics on top.  Operation order is fixed (no contraction;
+// Philox-keyed sampling, polarized scattering, absorption weighting and
+// peel-off images on top.  Operation order is fixed (no contraction;
 // explicit fma only) so that results match a CPU run of the same algorithm
 // bit for bit on the same packet ids.
 #pragma once

@@ -7,9 +7,10 @@ this package, as the checker or the timed CPU baseline.  The product
 The C sources restate the reference march of
 src/modules/particle/radiation/photon.h:20-125, types/proxy.h:64-121,
 types/particle_base.h:37-144 and mesh/geometry/geometry.h:68-83, plus the
-north-star physics (see kp_oracle.h for the list and the documented
-deviations), and the spherical march of radiation/photon_sph.h:10-266
-(kp_sph_oracle.c).
+physics the reference has no code for: Philox sampling, polarized
+scattering, absorption weighting, peel-off images (see kp_oracle.h for the
+list and the documented deviations), and the spherical march of
+radiation/photon_sph.h:10-266 (kp_sph_oracle.c).
 """
 from __future__ import annotations
 

@@ -11,7 +11,7 @@
  *    block reload of proc (photon.h:104-125), shift_blk level jumps
  *    (proxy.h:64-121), x_fc (geometry.h:68-83), periodic regulate
  *    (particle_base.h:37-56);
- *  - the north-star physics with no reference code, from published methods:
+ *  - the physics the reference has no code for, from published methods:
  *    Philox4x32-10 keyed by (seed, band) with counter (packet id, event),
  *    tau = -ln xi free flights, survival-biased absorption with Russian
  *    roulette, Thomson and Henyey-Greenstein/Rayleigh-form (White 1979)
