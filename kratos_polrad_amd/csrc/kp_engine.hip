@@ -1705,8 +1705,11 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 KP_HIP(hipEventElapsedTime(&ms, e0, e1));
                 return ms;
             };
+            // start from the persistent kind in slot order, whatever order an
+            // earlier tuning (of another precision, say) left behind
             kp_schedule cur = e->sched;
             cur.kind = KP_SCHED_PERSISTENT;
+            cur.coherent = 0;
             float best_ms = time_it(cur);
             // coordinate descent, one knob at a time
             auto try_knob = [&](int kp_schedule::*f, std::initializer_list<int> vals) {
@@ -1723,13 +1726,14 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 }
             };
             const bool sph_mesh = e->coord == KP_COORD_SPHERICAL;
-            // schedule kind and packet order together: lockstep histories pay
-            // off in emission-direction order.  FP64 persistent lanes refill
-            // at different times, lose the lockstep and then only collide on
-            // the same J addresses, so that pair is not a candidate; the
-            // mixed persistent kernel in that order refills whole waves and
-            // sums runs of lanes (wins on optically thin bands).  With
-            // observers the one-thread-per-packet kernels march every
+            // schedule kind and packet order together: lockstep histories
+            // pay off in emission-direction order.  FP64 persistent lanes in
+            // that order collide on the same J addresses, more so the denser
+            // the packets: a sample run can favour it where the full run
+            // loses, so that pair is not a candidate (callers may still set
+            // it).  The mixed persistent kernel in that order refills whole
+            // waves and sums runs of lanes (wins on optically thin bands).
+            // With observers the one-thread-per-packet kernels march every
             // peel-off ray inside the history, unbatched: not a candidate.
             for (int kind : {KP_SCHED_PERSISTENT, KP_SCHED_HISTORY})
                 for (int coh : {0, 1}) {
