@@ -228,21 +228,23 @@ def test_gpu_one_pass_replay_in_chunks(prec, vertex_slots):
 @pytest.mark.parametrize("model", [_abi.KP_HG_RAYLEIGH, _abi.KP_THOMSON], ids=["hg", "thomson"])
 def test_gpu_f32_one_pass_replay_several_bands(model):
     """Three bands with their own extinction scale, albedo and g (per-cell
-    or per-band) and uneven packet counts: the deferred kernels find each
-    record's band from its packet slot.  Same gradients as the two-pass
-    replay, for HG-Rayleigh and for Thomson scattering (no g terms)."""
+    or per-band) and uneven packet counts, and an empty band between them:
+    the deferred kernels find each record's band from its packet slot.  Same
+    gradients as the two-pass replay, for HG-Rayleigh and for Thomson
+    scattering (no g terms)."""
     import torch
 
     m, kappa, alb, g, run, W, sel = _setup(30000)
     bands = [BandSpec(n_packets=30000, cell_albedo=True, cell_g=True),
              BandSpec(n_packets=7000, kappa_scale=0.3, albedo=0.7, g=0.4),
+             BandSpec(n_packets=0, kappa_scale=0.5),
              BandSpec(n_packets=15000, kappa_scale=2.0, cell_albedo=True, g=0.1)]
     run = RunSpec(**{**run.__dict__, "bands": bands, "precision": _abi.KP_F32,
                      "scatter_model": model})
     eng = Engine(m)
     eng.set_media(kappa, alb, g)
     rng = np.random.default_rng(7)
-    w = torch.tensor(rng.normal(size=(3, 2, 16, 16, 3)), dtype=torch.float64, device="cuda")
+    w = torch.tensor(rng.normal(size=(4, 2, 16, 16, 3)), dtype=torch.float64, device="cuda")
     out = {}
     for q in (0, 1):
         eng.set_schedule(peel_queue=q)
@@ -257,3 +259,26 @@ def test_gpu_f32_one_pass_replay_several_bands(model):
         assert np.any(b != 0.0), k
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11 * np.abs(b).max(), err_msg=k)
     assert out[1]["jlin"][0] == pytest.approx(out[0]["jlin"][0], rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [_abi.KP_F32, _abi.KP_F64], ids=["f32", "f64"])
+def test_gpu_adjoint_without_packets_leaves_gradients(prec):
+    """A run whose bands hold no packets returns at once: the gradients and
+    J_lin handed in are left as they were."""
+    import torch
+
+    m, kappa, alb, g, run, W, sel = _setup(10)
+    run = RunSpec(**{**run.__dict__, "precision": prec,
+                     "bands": [BandSpec(n_packets=0, cell_albedo=True, cell_g=True)]})
+    eng = Engine(m)
+    eng.set_media(kappa, alb, g)
+    w = torch.tensor(W, dtype=torch.float64, device="cuda")
+    gr = {k: torch.full((m.n_cells,), 2.0, dtype=torch.float64, device="cuda")
+          for k in ("d_logrho", "d_albedo", "d_g")}
+    gr["jlin"] = torch.full((1,), 3.0, dtype=torch.float64, device="cuda")
+    eng.adjoint(SRC, run, w, grads=gr)
+    torch.cuda.synchronize()
+    for k in ("d_logrho", "d_albedo", "d_g"):
+        assert torch.all(gr[k] == 2.0), k
+    assert gr["jlin"].item() == 3.0
