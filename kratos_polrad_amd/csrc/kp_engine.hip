@@ -441,17 +441,19 @@ __global__ void __launch_bounds__(256) k_peel_keys(kp::kparams P, int32_t *keys)
 }
 
 // Gradient deposits of the queued peel-off rays, d/dlog(rho_c) += -dtau_c A_k
-// for every crossing: the same FP32 march, optical depth and cut-off as the
-// replay's own peel-off march, one thread per ray in image-plane order.
-template <class Pol> __global__ void __launch_bounds__(256) k_peel_grad(kp::kparams P)
+// for every crossing: the same march of the precision, optical depth and
+// cut-off as the replay's own peel-off march, one thread per ray in
+// image-plane order.  Blocks blk of nblk (grid-stride over the rays).
+template <class Pol>
+__device__ __forceinline__ void peel_grad_body(const kp::kparams &P, uint64_t blk, uint64_t nblk)
 {
     using real = typename Pol::real;
     const kp::peel_rec_t<real> *pq = static_cast<const kp::peel_rec_t<real> *>(P.pq);
     const uint64_t n = *P.pq_count < P.pq_cap ? *P.pq_count : (uint64_t)P.pq_cap;
     const int lane = threadIdx.x & 63;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t stride = nblk * blockDim.x;
     double *gl = kp::g_base(P, P.g_logrho);
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+    for (uint64_t base = blk * blockDim.x + (threadIdx.x & ~63u); base < n;
          base += stride) {
         const uint64_t k = base + lane;
         int64_t idx = -1;
@@ -523,15 +525,16 @@ template <class Pol> __global__ void __launch_bounds__(256) k_peel_grad(kp::kpar
 //                end: interaction, escape or step budget).
 // B_v / B_s: the A_k still ahead before / after the vertex's peel-offs.
 // The interaction's d/dlog(rho) term and the segment's first crossing (same
-// cell) go out as one atomic.
-template <class Pol> __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
+// cell) go out as one atomic.  Blocks blk of nblk.
+template <class Pol>
+__device__ __forceinline__ void vtx_grad_body(const kp::kparams &P, uint64_t blk, uint64_t nblk)
 {
     using real = typename Pol::real;
     const kp::vtx_rec_t<real> *vq = static_cast<const kp::vtx_rec_t<real> *>(P.vq);
     const uint64_t n = *P.vq_count < P.vq_cap ? *P.vq_count : (uint64_t)P.vq_cap;
     const int lane = threadIdx.x & 63;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+    const uint64_t stride = nblk * blockDim.x;
+    for (uint64_t base = blk * blockDim.x + (threadIdx.x & ~63u); base < n;
          base += stride) {
         const uint64_t k = base + lane;
         kp::vtx_rec_t<real> rec;
@@ -638,6 +641,23 @@ template <class Pol> __global__ void __launch_bounds__(256) k_vtx_grad(kp::kpara
         if (carry_gc >= 0)
             kp::atomic_add_f64(kp::g_base(P, P.g_logrho) + carry_gc, carry);
     }
+}
+
+template <class Pol> __global__ void __launch_bounds__(256) k_vtx_grad(kp::kparams P)
+{
+    vtx_grad_body<Pol>(P, blockIdx.x, gridDim.x);
+}
+
+// Both deferred deposits in one launch: the first n_peel blocks march the
+// peel-off rays (VALU-bound, runs of lanes summed), the others the vertex
+// records (bound by scattered atomics), so the two overlap on the device.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_defer_grad(kp::kparams P, uint32_t n_peel)
+{
+    if (blockIdx.x < n_peel)
+        peel_grad_body<Pol>(P, blockIdx.x, n_peel);
+    else
+        vtx_grad_body<Pol>(P, blockIdx.x - n_peel, gridDim.x - n_peel);
 }
 
 // One thread per packet, both replay passes in that thread (FP64).
@@ -1503,13 +1523,17 @@ kp_status kp_adjoint(kp_engine *e, const kp_source *src, const kp_run *run,
                 kp::sort_order(e->pq_keys, Pc.pq_cap, 30, e->pq_order, e->pq_ws, e->pq_ws_cap,
                                stream, e->pq_count);
                 Pc.pq_order = e->pq_order;
-                if (f64)
-                    hipLaunchKernelGGL(k_peel_grad<kp::pol64>, gq, tq, 0, stream, Pc);
-                else
-                    hipLaunchKernelGGL(k_peel_grad<kp::pol32>, gq, tq, 0, stream, Pc);
-                KP_HIP(hipGetLastError());
             }
-            if (vqueue) {
+            if (queue && vqueue) {
+                // peel-off rays and vertex records together
+                const uint32_t np = (uint32_t)(e->n_cu * 8);
+                const dim3 g2(2 * np);
+                if (f64)
+                    hipLaunchKernelGGL(k_defer_grad<kp::pol64>, g2, tq, 0, stream, Pc, np);
+                else
+                    hipLaunchKernelGGL(k_defer_grad<kp::pol32>, g2, tq, 0, stream, Pc, np);
+                KP_HIP(hipGetLastError());
+            } else if (vqueue) {
                 if (f64)
                     hipLaunchKernelGGL(k_vtx_grad<kp::pol64>, gq, tq, 0, stream, Pc);
                 else
