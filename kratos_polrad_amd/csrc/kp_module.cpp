@@ -224,6 +224,7 @@ struct kp_module {
     int coord = KP_COORD_CARTESIAN;
     double r_ratio = 1.0;
     bool sched_given = false;
+    int order = -1; // photon|order: -1 = the engine's default, 0 slot, 1 direction
     // [particle]
     int n_hv = 1, n_cyc = 1, n_th = 64, step_max = 100;
     int64_t n_par = 1;
@@ -304,6 +305,15 @@ struct kp_module {
         const std::string sk = ini.get<std::string>("photon", "schedule", "persistent");
         sched_kind = sk == "history" ? KP_SCHED_HISTORY : KP_SCHED_PERSISTENT;
         sched_given = ini.has("photon", "schedule");
+        if (ini.has("photon", "order")) {
+            const std::string o = ini.get<std::string>("photon", "order", "slot");
+            if (o == "slot")
+                order = 0;
+            else if (o == "direction")
+                order = 1;
+            else
+                throw kp::error(KP_ERR_ARG, "ini: photon|order must be slot or direction");
+        }
         p_max = ini.get<double>("photon", "p_max", 1.0);
         kappa = ini.get<double>("photon", "kappa", 0.0);
         weight = ini.get<double>("photon", "weight", 1.0);
@@ -391,6 +401,8 @@ struct kp_module {
             s.kind = sched_kind;
         if (s.kind == KP_SCHED_HISTORY)
             s.block_threads = 256;
+        if (order >= 0)
+            s.coherent = order;
         st = kp_set_schedule(eng, &s);
         if (st != KP_OK)
             throw kp::error(st, kp_last_error());

@@ -65,6 +65,7 @@ def test_read_keys_and_defaults():
      "indivisible"),
     (INI.replace("model = hg_rayleigh", "model = mie"), "model"),
     (INI.replace("albedo = 0.6 0.8", "albedo = 0.6 0.8 0.9"), "albedo"),
+    (INI + "order = random\n", "order"),
 ])
 def test_read_rejects_bad_input(bad, msg):
     with pytest.raises(_abi.KPError) as e:
@@ -134,10 +135,13 @@ kappa = 0.25
 
 
 @pytest.mark.gpu
-def test_step_equals_forward_runs(tmp_path):
+@pytest.mark.parametrize("order", ["", "order = direction\n"], ids=["default", "direction"])
+def test_step_equals_forward_runs(tmp_path, order):
+    """photon|order = direction marches each launch's packets in
+    emission-direction order: same FP64 histories, same tallies."""
     import torch
 
-    m = PhotonModule(INI).init(device=0)
+    m = PhotonModule(INI + order).init(device=0)
     m.step()
     got = m.tallies()
     assert m.info()["packets_done"] == 6000
