@@ -40,7 +40,8 @@
 namespace kp {
 
 enum : int { M_NEED = 0, M_MAIN = 1, M_PEEL = 2, M_DONE = 3, M_PEND = 4 };
-enum : int { PA_NONE = 0, PA_INTERACT = 1, PA_PEEL_END = 2, PA_SCATTER = 3, PA_FINISH = 4 };
+enum : int { PA_NONE = 0, PA_INTERACT = 1, PA_PEEL_END = 2, PA_SCATTER = 3, PA_FINISH = 4,
+             PA_BLOCK = 5 };
 enum : int { EV_EMIT = 0, EV_SCATTER = 1 };
 
 // ---------------------------------------------------------------- policies
@@ -59,6 +60,7 @@ struct pol64 {
     }
     __device__ static void relocate(const kparams &P, ray &r) { f64::gl_load(P, r); }
     __device__ static void set_dir(const kparams &, ray &, const real *) {}
+    static constexpr bool kDeferBlock = false; // block changes inside the march step
     __device__ static int dl(const ray &r, real &l) { return f64::get_dl(r, l); }
     __device__ static int cross(const kparams &P, ray &r, int ap, real l)
     {
@@ -104,10 +106,20 @@ struct pol32 {
     }
     __device__ static void relocate(const kparams &P, ray &r) { f32::set_scales(P, r); }
     __device__ static void set_dir(const kparams &P, ray &r, const real *) { f32::set_scales(P, r); }
+    // block changes wait for a block phase of the wave (block_stage)
+    static constexpr bool kDeferBlock = true;
     __device__ static int dl(const ray &r, real &l) { return f32::get_dl(r, l); }
     __device__ static int cross(const kparams &P, ray &r, int ap, real l)
     {
         return f32::cross_face(P, r, ap, l);
+    }
+    __device__ static int cross_step(const kparams &P, ray &r, int ap, real l)
+    {
+        return f32::cross_in_block(P, r, ap, l);
+    }
+    __device__ static int change_block(const kparams &P, ray &r, int a, int side)
+    {
+        return f32::change_block(P, r, a, side);
     }
     __device__ static void advance(ray &r, real l)
     {
@@ -157,6 +169,7 @@ template <class Pol> struct lane_t {
     int band;
     int mode, ev_kind, o;
     int pa, status;   // pending action and, for PA_FINISH, the end status
+    int blk;          // PA_BLOCK: exit axis + 4 * face side + 8 * the march mode to resume
     int64_t gc;       // cell of the pending / current interaction
     int64_t kgc;      // cell whose kappa_rho is in kcache (issued one step ahead)
     float kcache;
@@ -794,6 +807,31 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P)
     }
 }
 
+// Block change of a lane that left its block (PA_BLOCK), run for the
+// waiting lanes of a wave together: the neighbour-table lookups and the
+// level rescale are issued once per block phase instead of in nearly every
+// march step.  The lane resumes the march it left, with the next cell's
+// kappa_rho load issued, or ends that march as the march step would have.
+template <class Pol> __device__ __forceinline__ void block_stage(const kparams &P, lane_t<Pol> &L)
+{
+    const int mode = L.blk >> 3;
+    const int st = Pol::change_block(P, L.cur, L.blk & 3, (L.blk >> 2) & 1);
+    if (st == f32::ST_IN) {
+        L.mode = mode;
+        L.pa = PA_NONE;
+        L.kgc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);
+        L.kcache = P.kappa[L.kgc];
+    } else if (mode == M_MAIN) {
+        L.pa = PA_FINISH;
+        L.status = (st == f32::ST_ESCAPED) ? KP_END_ESCAPED : KP_END_LOST;
+    } else {
+        L.pa = PA_PEEL_END;
+    }
+}
+
+// Waiting block changes that start a block phase of their own.
+constexpr uint32_t kBlockBatch = 8;
+
 // lds: the launch gives each workgroup the kp_lds accumulators.
 // SYNC (mixed forward in emission-direction order): a wave takes new packets
 // only when all its lanes are idle, so lanes that start together fly
@@ -805,9 +843,13 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                              double &jsum, bool lds)
 {
     using real = typename Pol::real;
+    // forward launches defer block changes (the replays keep them in the
+    // march step: their larger lane state would spill)
+    constexpr bool kDB = Pol::kDeferBlock && !ADJ;
     lane_t<Pol> L;
     L.mode = M_NEED;
     L.pa = PA_NONE;
+    L.blk = 0;
     L.kgc = -1;
     L.kcache = 0.0f;
     L.pass = 1;
@@ -850,10 +892,23 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         const uint64_t live = __ballot(L.mode != M_DONE);
         if (!live)
             break;
-        const uint64_t wait = __ballot(L.mode == M_PEND || L.mode == M_NEED);
+        // lanes waiting for a block change do not count towards an action
+        // phase; they get a block phase when enough of them wait, when no
+        // lane marches, or with an action phase (SYNC: at once, so the lanes
+        // stay in step)
+        const bool bwait = kDB && L.mode == M_PEND && L.pa == PA_BLOCK;
+        const uint64_t wait = __ballot((L.mode == M_PEND && !bwait) || L.mode == M_NEED);
         const uint64_t march = __ballot(L.mode == M_MAIN || L.mode == M_PEEL);
         const uint32_t nw = __popcll(wait);
-        if (wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march)) {
+        const bool phase = wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march);
+        if constexpr (kDB) {
+            const uint64_t bw = __ballot(bwait);
+            if (bw && (SYNC || phase || !march || (uint32_t)__popcll(bw) >= kBlockBatch)) {
+                if (bwait)
+                    block_stage<Pol>(P, L);
+            }
+        }
+        if (phase) {
             // ---------------- action phase: every pending lane together
             if (L.mode == M_PEND && L.pa == PA_PEEL_END)
                 peel_end_stage<Pol, ADJ, DEFER>(P, L, lds);
@@ -986,7 +1041,18 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                 continue;
             }
         }
-        const int st = Pol::cross(P, L.cur, ap, l);
+        int st;
+        if constexpr (kDB) {
+            st = Pol::cross_step(P, L.cur, ap, l);
+            if (st != f32::ST_IN) {
+                L.blk = ap + 4 * (st == f32::ST_BLOCK_HI) + 8 * L.mode;
+                L.mode = M_PEND;
+                L.pa = PA_BLOCK;
+                continue;
+            }
+        } else {
+            st = Pol::cross(P, L.cur, ap, l);
+        }
         if (st == f64::ST_IN) {
             // issue the next cell's kappa_rho load now; it is consumed next step
             L.kgc = (int64_t)L.cur.ib * P.n3 + Pol::cell(P, L.cur);

@@ -86,14 +86,15 @@ __device__ __forceinline__ int get_dl(const ray_t &r, float &lmin)
     return ap;
 }
 
-enum { ST_IN = 0, ST_ESCAPED = 1, ST_LOST = 4 };
+// ST_BLOCK_LO / ST_BLOCK_HI: the index left [0,n) through the lower / upper
+// face of the exit axis; the ray changes block (change_block) before it
+// marches on.
+enum { ST_IN = 0, ST_ESCAPED = 1, ST_BLOCK_LO = 2, ST_BLOCK_HI = 3, ST_LOST = 4 };
 
-// Step to the exit face of axis ap after physical length l, then change
-// block through the face-neighbour table if the index left [0,n).
-__device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, float l)
+// Step to the exit face of axis ap after physical length l.
+__device__ __forceinline__ int cross_in_block(const kparams &P, ray_t &r, int ap, float l)
 {
     const int n = P.n_cell;
-    const float nf = (float)n;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (a == ap) {
@@ -104,13 +105,19 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, fl
             r.x[a] = fmaf(l, r.cpl[a], r.x[a]);
         }
     }
-    const int a = ap;
     // runtime axis indices are resolved with unrolled selects so the ray
     // stays in registers
-    const int ia = a == 0 ? r.i[0] : (a == 1 ? r.i[1] : r.i[2]);
-    if (ia >= 0 && ia < n)
-        return ST_IN;
-    const int side = ia > 0;
+    const int ia = ap == 0 ? r.i[0] : (ap == 1 ? r.i[1] : r.i[2]);
+    return (ia >= 0 && ia < n) ? ST_IN : (ia > 0 ? ST_BLOCK_HI : ST_BLOCK_LO);
+}
+
+// Block change of a ray whose index left [0,n) through the exit face of axis
+// a (side 1: the upper face): through the face-neighbour table to the
+// neighbour block, with the position rescaled across a level jump.
+__device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, int side)
+{
+    const int n = P.n_cell;
+    const float nf = (float)n;
     if (P.phybnd[r.ib] & (1u << (2 * a + side)))
         return ST_ESCAPED;
     const int ap0 = a == 2 ? 0 : a + 1;
@@ -136,12 +143,13 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, fl
     r.ib = nb;
     const float half = (float)(n / 2);
     const float sc = dl < 0 ? 2.0f : (dl > 0 ? 0.5f : 1.0f);
+    const float isc = dl < 0 ? 0.5f : (dl > 0 ? 2.0f : 1.0f); // 1 / sc, exact
     // cell volumes of neighbouring levels differ by exactly 8
     r.iv *= dl < 0 ? 0.125f : (dl > 0 ? 8.0f : 1.0f);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         r.lpc[q] *= sc;
-        r.cpl[q] *= 1.0f / sc;
+        r.cpl[q] *= isc;
         if (q == a) {
             r.i[q] = side ? 0 : n - 1;
             r.x[q] = side ? 0.0f : nf;
@@ -156,6 +164,12 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, fl
             r.i[q] = min(max((int)floorf(r.x[q]), 0), n - 1);
     }
     return ST_IN;
+}
+
+__device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, float l)
+{
+    const int st = cross_in_block(P, r, ap, l);
+    return st == ST_IN ? ST_IN : change_block(P, r, ap, st == ST_BLOCK_HI);
 }
 
 __device__ __forceinline__ int cell_index(const kparams &P, const ray_t &r)
