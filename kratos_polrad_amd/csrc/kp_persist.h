@@ -39,6 +39,11 @@
 
 namespace kp {
 
+// the march code reads the statuses of both precisions by one set of names
+static_assert(f32::ST_IN == f64::ST_IN && f32::ST_ESCAPED == f64::ST_ESCAPED &&
+                  f32::ST_BLOCK_LO == f64::ST_BLOCK_LO && f32::ST_BLOCK_HI == f64::ST_BLOCK_HI &&
+                  f32::ST_LOST == f64::ST_LOST,
+              "FP32 and FP64 crossing statuses differ");
 enum : int { M_NEED = 0, M_MAIN = 1, M_PEEL = 2, M_DONE = 3, M_PEND = 4 };
 enum : int { PA_NONE = 0, PA_INTERACT = 1, PA_PEEL_END = 2, PA_SCATTER = 3, PA_FINISH = 4,
              PA_BLOCK = 5 };
@@ -60,12 +65,29 @@ struct pol64 {
     }
     __device__ static void relocate(const kparams &P, ray &r) { f64::gl_load(P, r); }
     __device__ static void set_dir(const kparams &, ray &, const real *) {}
-    static constexpr bool kDeferBlock = false; // block changes inside the march step
+    // block changes wait for a block phase of the wave (block_stage)
+    static constexpr bool kDeferBlock = true;
     __device__ static int dl(const ray &r, real &l) { return f64::get_dl(r, l); }
     __device__ static int cross(const kparams &P, ray &r, int ap, real l)
     {
         bool rl;
         const int st = f64::cross_face(P, r, ap, l, rl);
+        if (st == f64::ST_IN && rl)
+            f64::gl_load(P, r);
+        return st;
+    }
+    __device__ static int cross_step(const kparams &P, ray &r, int ap, real l)
+    {
+        bool rl;
+        const int st = f64::cross_in_block(P, r, ap, l, rl);
+        if (st == f64::ST_IN && rl)
+            f64::gl_load(P, r);
+        return st;
+    }
+    __device__ static int change_block(const kparams &P, ray &r, int a, int side)
+    {
+        bool rl = false;
+        const int st = f64::change_block(P, r, a, side, rl);
         if (st == f64::ST_IN && rl)
             f64::gl_load(P, r);
         return st;

@@ -20,7 +20,9 @@ struct ray_t {
     int ib;
 };
 
-enum { ST_IN = 0, ST_ESCAPED = 1, ST_LOST = 4 };
+// ST_BLOCK_LO / ST_BLOCK_HI: the index left [0,n) through the lower / upper
+// face of the exit axis (cross_in_block); change_block takes the ray on.
+enum { ST_IN = 0, ST_ESCAPED = 1, ST_BLOCK_LO = 2, ST_BLOCK_HI = 3, ST_LOST = 4 };
 
 __device__ __forceinline__ double dot3(const double *a, const double *b)
 {
@@ -78,9 +80,32 @@ __device__ __forceinline__ int get_dl(const ray_t &r, double &lmin)
 }
 
 // proc_geo (photon.h:54-71) + shift_blk (proxy.h:64-121) + regulate
-// (particle_base.h:37-56).
-__device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, double l,
-                                          bool &reload)
+// (particle_base.h:37-56), in three parts that keep the reference's order:
+// the exit-axis step (cross_in_block), the block change when the index left
+// [0,n) (change_block), then the periodic wrap (regulate).
+__device__ __forceinline__ void regulate(const kparams &P, ray_t &r, int a, bool &reload)
+{
+    // periodic wrap (par_base_t::regulate); a ray that just left through the
+    // lower domain face sits exactly on it and belongs to the upper image
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if (P.periodic[q]) {
+            const double lbox = P.xmax[q] - P.xmin[q];
+            const bool down_exit = (q == a) && !(r.d[q] > 0.0);
+            if (r.x[q] < P.xmin[q] || (down_exit && r.x[q] == P.xmin[q])) {
+                r.x[q] += lbox;
+                reload = true;
+            } else if (r.x[q] >= P.xmax[q]) {
+                r.x[q] -= lbox;
+                reload = true;
+            }
+        }
+}
+
+// Step to the exit face of axis ap after physical length l: ST_IN (wrapped
+// periodically), or ST_BLOCK_LO / ST_BLOCK_HI.
+__device__ __forceinline__ int cross_in_block(const kparams &P, ray_t &r, int ap, double l,
+                                              bool &reload)
 {
     reload = false;
 #pragma unroll
@@ -101,69 +126,70 @@ __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, do
         }
     }
     const int n = P.n_cell;
-    const int a = ap; // only the exit axis can leave [0,n)
     // runtime axis indices are resolved with unrolled selects so the ray
     // stays in registers
-    const int ia = a == 0 ? r.i[0] : (a == 1 ? r.i[1] : r.i[2]);
-    if (!(ia >= 0 && ia < n)) {
-        const int side = ia > 0;
-        if (P.phybnd[r.ib] & (1u << (2 * a + side)))
-            return ST_ESCAPED;
-        const int ap0 = a == 2 ? 0 : a + 1;
-        const int dl = P.d_lvl[6 * r.ib + 2 * a + side];
-        int j0 = 0, j1 = 0;
-        if (dl > 0) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (q == a)
-                    continue;
-                const int jj = r.i[q] >= n / 2;
-                if (q == ap0)
-                    j0 = jj;
-                else
-                    j1 = jj;
-            }
-        }
-        const int slot = 24 * r.ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
-        const int nb = P.nb_id[slot];
-        if (nb < 0)
-            return ST_LOST;
-        const int shift = P.nb_shift[slot];
-        r.ib = nb;
+    const int ia = ap == 0 ? r.i[0] : (ap == 1 ? r.i[1] : r.i[2]);
+    if (!(ia >= 0 && ia < n))
+        return ia > 0 ? ST_BLOCK_HI : ST_BLOCK_LO;
+    regulate(P, r, ap, reload);
+    return ST_IN;
+}
+
+// Block change of a ray whose index left [0,n) through the exit face of axis
+// a (side 1: the upper face), then the periodic wrap; reload is set.
+__device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, int side,
+                                            bool &reload)
+{
+    const int n = P.n_cell;
+    if (P.phybnd[r.ib] & (1u << (2 * a + side)))
+        return ST_ESCAPED;
+    const int ap0 = a == 2 ? 0 : a + 1;
+    const int dl = P.d_lvl[6 * r.ib + 2 * a + side];
+    int j0 = 0, j1 = 0;
+    if (dl > 0) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            if (q == a) {
-                r.i[q] = side ? 0 : n - 1;
+            if (q == a)
                 continue;
-            }
-            const int j = (q == ap0) ? 0 : 1;
-            if (dl < 0) {
-                r.i[q] = r.i[q] / 2 + ((shift >> j) & 1) * (n / 2);
-            } else if (dl > 0) {
-                int f = (r.i[q] < n / 2 ? r.i[q] : r.i[q] - n / 2) * 2;
-                if (r.x[q] >= x_fc(P, nb, q, f + 1))
-                    f += 1;
-                r.i[q] = f;
-            }
+            const int jj = r.i[q] >= n / 2;
+            if (q == ap0)
+                j0 = jj;
+            else
+                j1 = jj;
         }
-        reload = true;
     }
-    // periodic wrap (par_base_t::regulate); a ray that just left through the
-    // lower domain face sits exactly on it and belongs to the upper image
+    const int slot = 24 * r.ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
+    const int nb = P.nb_id[slot];
+    if (nb < 0)
+        return ST_LOST;
+    const int shift = P.nb_shift[slot];
+    r.ib = nb;
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-        if (P.periodic[q]) {
-            const double lbox = P.xmax[q] - P.xmin[q];
-            const bool down_exit = (q == a) && !(r.d[q] > 0.0);
-            if (r.x[q] < P.xmin[q] || (down_exit && r.x[q] == P.xmin[q])) {
-                r.x[q] += lbox;
-                reload = true;
-            } else if (r.x[q] >= P.xmax[q]) {
-                r.x[q] -= lbox;
-                reload = true;
-            }
+    for (int q = 0; q < 3; ++q) {
+        if (q == a) {
+            r.i[q] = side ? 0 : n - 1;
+            continue;
         }
+        const int j = (q == ap0) ? 0 : 1;
+        if (dl < 0) {
+            r.i[q] = r.i[q] / 2 + ((shift >> j) & 1) * (n / 2);
+        } else if (dl > 0) {
+            int f = (r.i[q] < n / 2 ? r.i[q] : r.i[q] - n / 2) * 2;
+            if (r.x[q] >= x_fc(P, nb, q, f + 1))
+                f += 1;
+            r.i[q] = f;
+        }
+    }
+    reload = true;
+    regulate(P, r, a, reload);
     return ST_IN;
+}
+
+__device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, double l,
+                                          bool &reload)
+{
+    const int st = cross_in_block(P, r, ap, l, reload);
+    return st == ST_IN ? ST_IN : change_block(P, r, ap, st == ST_BLOCK_HI, reload);
 }
 
 __device__ __forceinline__ int cell_index(const kparams &P, const ray_t &r)
