@@ -852,7 +852,7 @@ template <class Pol> __device__ __forceinline__ void block_stage(const kparams &
 }
 
 // Waiting block changes that start a block phase of their own.
-constexpr uint32_t kBlockBatch = 8;
+constexpr uint32_t kBlockBatch = 16;
 
 // lds: the launch gives each workgroup the kp_lds accumulators.
 // SYNC (mixed forward in emission-direction order): a wave takes new packets
