@@ -32,7 +32,7 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 # 5e6 packets: 28.05 GB fetched, after scaling by the 2.000 a 512 MiB copy
 # calls for, plus 24.57 GB written = 52.62 GB, against 30.35 GB algorithmic;
 # profiles/k_persist_pol32_config2_amr3.md).  Both scale with the packets.
-TRAFFIC_PER_ALG_BYTE = {"config2_amr3_r2_lognormal_hg": 52.62 / 30.35}
+TRAFFIC_PER_ALG_BYTE = {"config2_amr3_r2_lognormal_hg": 52.65 / 30.34}
 T_START = time.perf_counter()
 
 
