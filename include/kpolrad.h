@@ -246,7 +246,10 @@ typedef struct kp_schedule {
     int32_t j_replicas;
     int32_t img_replicas;
     /* persistent kernel: a wave runs its pending interactions / peel-off
-     * ends / packet ends together once this many lanes wait (1..64) */
+     * ends / packet ends together once this many lanes wait (1..64).
+     * Forward launches also park lanes whose ray leaves its block and run
+     * their block changes together (16 waiting, no lane marching, or with
+     * such a phase); this does not change any history. */
     int32_t event_batch;
     /* register budget of the persistent kernel: minimum waves per SIMD the
      * compiler must fit (0 = default: FP32 4, FP64 unconstrained; 1 =
