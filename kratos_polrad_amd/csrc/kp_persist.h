@@ -865,9 +865,11 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
                              double &jsum, bool lds)
 {
     using real = typename Pol::real;
-    // forward launches defer block changes (the replays keep them in the
-    // march step: their larger lane state would spill)
-    constexpr bool kDB = Pol::kDeferBlock && !ADJ;
+    // forward launches defer block changes; the replays keep them in the
+    // march step (their larger lane state would spill), and so does the
+    // lockstep variant (its waves would wait on each parked lane's
+    // un-prefetched kappa_rho load)
+    constexpr bool kDB = Pol::kDeferBlock && !ADJ && !SYNC;
     lane_t<Pol> L;
     L.mode = M_NEED;
     L.pa = PA_NONE;
@@ -916,8 +918,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
             break;
         // lanes waiting for a block change do not count towards an action
         // phase; they get a block phase when enough of them wait, when no
-        // lane marches, or with an action phase (SYNC: at once, so the lanes
-        // stay in step)
+        // lane marches, or with an action phase
         const bool bwait = kDB && L.mode == M_PEND && L.pa == PA_BLOCK;
         const uint64_t wait = __ballot((L.mode == M_PEND && !bwait) || L.mode == M_NEED);
         const uint64_t march = __ballot(L.mode == M_MAIN || L.mode == M_PEEL);
@@ -925,7 +926,7 @@ __device__ void persist_body(const kparams &P, unsigned long long *work, lane_co
         const bool phase = wait && (nw >= P.batch || 2 * nw >= (uint32_t)__popcll(live) || !march);
         if constexpr (kDB) {
             const uint64_t bw = __ballot(bwait);
-            if (bw && (SYNC || phase || !march || (uint32_t)__popcll(bw) >= kBlockBatch)) {
+            if (bw && (phase || !march || (uint32_t)__popcll(bw) >= kBlockBatch)) {
                 if (bwait)
                     block_stage<Pol>(P, L);
             }
