@@ -151,8 +151,11 @@ void kp_engine_destroy(kp_engine *eng);
 
 /* Per-cell medium, host arrays of n_blocks*n_cell^3 floats in the block
  * table's cell order: kappa_rho (extinction per unit length, the rad_t
- * mfp_i field of rad_data.h:16), albedo, HG asymmetry g (may be NULL:
- * then 0).  Copied to the device. */
+ * mfp_i field of rad_data.h:16) >= 0, albedo in [0,1], HG asymmetry g in
+ * (-1,1) (may be NULL: then 0).  Blocking: waits for all work already
+ * issued on the engine's device (any stream, graph replays included), then
+ * copies the medium in place, so earlier launches read the old medium and
+ * later ones the new. */
 kp_status kp_set_media(kp_engine *eng, const float *kappa_rho,
                        const float *albedo, const float *g);
 

@@ -800,6 +800,10 @@ void upload_bands(kp_engine *e, const kp_run *run, uint64_t &total, hipStream_t 
         const kp_band &s = run->bands[b];
         if (!(s.kappa_scale >= 0.0))
             throw kp::error(KP_ERR_ARG, "kp_forward: band kappa_scale must be >= 0");
+        if (!(s.flags & KP_BAND_CELL_ALBEDO) && !(s.albedo >= 0.0 && s.albedo <= 1.0))
+            throw kp::error(KP_ERR_ARG, "kp_forward: band albedo must be in [0,1]");
+        if (!(s.flags & KP_BAND_CELL_G) && !(s.g > -1.0 && s.g < 1.0))
+            throw kp::error(KP_ERR_ARG, "kp_forward: band g must be in (-1,1)");
         db[b].kappa_scale = s.kappa_scale;
         db[b].albedo = s.albedo;
         db[b].g = s.g;
@@ -1254,6 +1258,13 @@ kp_status kp_set_media(kp_engine *e, const float *kappa_rho, const float *albedo
             if (!(kappa_rho[q] >= 0.0f) || !(albedo[q] >= 0.0f && albedo[q] <= 1.0f))
                 throw kp::error(KP_ERR_ARG, "kp_set_media: kappa_rho must be >= 0 and "
                                             "albedo in [0,1]");
+        if (g)
+            for (size_t q = 0; q < nc; ++q)
+                if (!(g[q] > -1.0f && g[q] < 1.0f))
+                    throw kp::error(KP_ERR_ARG, "kp_set_media: g must be in (-1,1)");
+        // launches and graph replays already issued on any stream of the
+        // device read the old medium: wait for them before overwriting it
+        KP_HIP(hipDeviceSynchronize());
         if (!e->kappa) {
             e->kappa = dmalloc<float>(nc);
             e->albedo = dmalloc<float>(nc);
@@ -1266,6 +1277,8 @@ kp_status kp_set_media(kp_engine *e, const float *kappa_rho, const float *albedo
         } else {
             KP_HIP(hipMemset(e->g, 0, nc * sizeof(float)));
         }
+        // later launches on any stream see the whole new medium
+        KP_HIP(hipDeviceSynchronize());
         e->media_set = true;
     });
 }
@@ -1664,6 +1677,9 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
         check_forward_args(e, src, run, &probe_shape, "kp_autotune");
         if (!best || sample_packets < 1)
             throw kp::error(KP_ERR_ARG, "kp_autotune: need best and sample_packets >= 1");
+        // the sample runs always render the images
+        if (run->n_incl > 0 && (run->image_n < 1 || !(run->image_half > 0.0)))
+            throw kp::error(KP_ERR_ARG, "kp_autotune: bad image geometry");
         device_guard dg(e->device);
         hipStream_t stream = (hipStream_t)stream_;
         uint64_t total = 0;

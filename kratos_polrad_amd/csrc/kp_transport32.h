@@ -225,7 +225,8 @@ __device__ __forceinline__ long pixel(const kparams &P, int band, int o, const f
     const float half = (float)P.image_half, sc = (float)P.pix_scale;
     const float fx = floorf((uh + half) * sc);
     const float fy = floorf((uv + half) * sc);
-    if (fx < 0.0f || fy < 0.0f || fx >= (float)N || fy >= (float)N)
+    // written so that a NaN coordinate is outside too
+    if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)N && fy < (float)N))
         return -1;
     return (long)((((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
 }

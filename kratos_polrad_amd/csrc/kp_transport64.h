@@ -237,7 +237,8 @@ __device__ __forceinline__ long pixel(const kparams &P, int band, int o, const d
     const double uv = dot3(rel, P.ev[o]);
     const double fx = floor((uh + P.image_half) * P.pix_scale);
     const double fy = floor((uv + P.image_half) * P.pix_scale);
-    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+    // written so that a NaN coordinate is outside too
+    if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)N && fy < (double)N))
         return -1;
     return (long)((((size_t)band * P.n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
 }

@@ -314,7 +314,7 @@ static void deposit(const fctx_t *c, fthr_t *t, int band, int o, const double *x
     const double uv = dot3(rel, c->ev[o]);
     const double fx = floor((uh + c->run->image_half) * c->pix_scale);
     const double fy = floor((uv + c->run->image_half) * c->pix_scale);
-    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+    if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)N && fy < (double)N))
         return;
     const size_t pix = (((size_t)band * c->n_incl + o) * N + (size_t)fy) * N + (size_t)fx;
     t->image[3 * pix + 0] += I;
@@ -774,7 +774,7 @@ static long pix_index(const fctx_t *c, int band, int o, const double *x)
     const double uv = dot3(rel, c->ev[o]);
     const double fx = floor((uh + c->run->image_half) * c->pix_scale);
     const double fy = floor((uv + c->run->image_half) * c->pix_scale);
-    if (fx < 0.0 || fy < 0.0 || fx >= (double)N || fy >= (double)N)
+    if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)N && fy < (double)N))
         return -1;
     return (long)((((size_t)band * c->n_incl + o) * N + (size_t)fy) * N + (size_t)fx);
 }

@@ -116,6 +116,39 @@ def test_autograd_function_uses_the_adjoint():
 
 
 @pytest.mark.gpu
+def test_autograd_two_images_each_use_their_own_medium():
+    """Two images of different media in one loss: each backward replays its
+    own medium (the engine holds the second one when backward starts)."""
+    import torch
+
+    from kratos_polrad_amd.autograd import polarized_image
+
+    m, kappa, alb, g, run, W, sel = _setup(20000)
+    eng = Engine(m)
+    rng = np.random.default_rng(21)
+    lr1 = torch.zeros(m.n_cells, dtype=torch.float64, requires_grad=True)
+    lr2 = torch.tensor(0.3 * rng.normal(size=m.n_cells), dtype=torch.float64,
+                       requires_grad=True)
+    a = torch.tensor(alb, dtype=torch.float64, requires_grad=True)
+    gg = torch.tensor(g, dtype=torch.float64)
+    w1 = torch.tensor(W, dtype=torch.float64)
+    w2 = torch.tensor(rng.normal(size=W.shape), dtype=torch.float64)
+    img1 = polarized_image(eng, SRC, run, kappa, lr1, a, gg)
+    img2 = polarized_image(eng, SRC, run, kappa, lr2, a, gg)
+    ((img1 * w1).sum() + (img2 * w2).sum()).backward()
+    run64 = RunSpec(**{**run.__dict__, "precision": _abi.KP_F64})
+    want_a = torch.zeros(m.n_cells, dtype=torch.float64)
+    for lr, w in ((lr1, w1), (lr2, w2)):
+        k = (kappa.astype(np.float64) * np.exp(lr.detach().numpy())).astype(np.float32)
+        eng.set_media(k, alb, g)
+        ref = eng.adjoint(SRC, run64, w.cuda())
+        torch.cuda.synchronize()
+        torch.testing.assert_close(lr.grad, ref["d_logrho"].cpu(), rtol=1e-9, atol=1e-12)
+        want_a += ref["d_albedo"].cpu()
+    torch.testing.assert_close(a.grad, want_a, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.gpu
 def test_gpu_f32_adjoint_tracks_f64_adjoint():
     """The FP32 replay draws the same Philox words: its region gradients agree
     with the FP64 replay far inside the Monte Carlo noise (0.5 %)."""
