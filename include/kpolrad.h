@@ -373,6 +373,21 @@ kp_status kp_bin_by_key(const int32_t *keys, int64_t n, int32_t n_bins, int64_t 
                         int32_t *order, int32_t *slot, void *workspace,
                         size_t workspace_bytes, void *stream);
 
+/* Largest n that kp_bin_by_key_static takes. */
+#define KP_BIN_STATIC_MAX (1 << 21)
+
+/* kp_bin_by_key that allocates nothing and takes no workspace: for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Its workspace is a fixed device array inside the library,
+ * about 34 MB on each GPU it runs on, held until the process ends; n above
+ * KP_BIN_STATIC_MAX is rejected with KP_ERR_ARG.  Every call on one GPU
+ * uses that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialized by the caller.  Results equal
+ * kp_bin_by_key's on the same input. */
+kp_status kp_bin_by_key_static(const int32_t *keys, int64_t n, int32_t n_bins,
+                               int64_t *offsets, int32_t *order, int32_t *slot, void *stream);
+
 /* comm_t::cpy_d (boundary/comm.h:226-239): dst[s] = src[order[s]] for
  * s < min(*n_dev, n_max), records of unit_bytes (multiple of 4); n_dev is a
  * device int64 (e.g. offsets + n_bins of kp_bin_by_key). */

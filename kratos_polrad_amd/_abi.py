@@ -183,6 +183,7 @@ SIGNATURES = {
     "kp_module_mesh": (C.c_int, [_P, C.POINTER(_P)]),
     "kp_pool_keep": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, C.c_size_t, _P]),
     "kp_bin_by_key": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_size_t, _P]),
+    "kp_bin_by_key_static": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, _P]),
     "kp_gather": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, _P, _P]),
     "kp_graph_destroy": (None, [_P]),
     "kp_get_schedule": (C.c_int, [_P, C.POINTER(Schedule)]),

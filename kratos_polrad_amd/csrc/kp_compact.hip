@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -132,7 +133,7 @@ struct scan_ws {
     uint64_t n_chunks;
 };
 
-uint64_t scan_chunks(uint64_t n) { return (n + kScanChunk - 1) / kScanChunk; }
+constexpr uint64_t scan_chunks(uint64_t n) { return (n + kScanChunk - 1) / kScanChunk; }
 
 void exscan(uint32_t *data, uint64_t n, const scan_ws &ws, uint32_t *total, hipStream_t s)
 {
@@ -378,7 +379,7 @@ __global__ void __launch_bounds__(kThreads) k_gather(const W *src, uint32_t word
 }
 
 // ------------------------------------------------------------ workspace
-size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct keep_layout {
     size_t tile_cnt, sums, scalars, hole_at, mover_at, total;
@@ -405,7 +406,7 @@ keep_layout keep_ws(uint64_t n)
 struct bin_layout {
     size_t ka, kb, ia, ib, hist, sums, total;
 };
-bin_layout bin_ws(uint64_t n)
+constexpr bin_layout bin_ws(uint64_t n)
 {
     const uint64_t nt = (n + kTile - 1) / kTile;
     bin_layout L{};
@@ -456,6 +457,107 @@ __global__ void __launch_bounds__(kThreads) k_copy_order(const uint32_t *sidx, u
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
         order[p] = (int32_t)sidx[p];
+}
+
+// Body of kp_bin_by_key on a workspace of ws_bytes at ws (device memory).
+void bin_by_key_run(const char *who, const int32_t *keys, int64_t n, int32_t n_bins,
+                    int64_t *offsets, int32_t *order, int32_t *slot, void *ws, size_t ws_bytes,
+                    hipStream_t s)
+{
+    const std::string w(who);
+    if (n < 0 || n >= (1ll << 31))
+        throw kp::error(KP_ERR_ARG, w + ": n must be in [0, 2^31)");
+    if (n_bins < 1 || n_bins >= (1 << 30))
+        throw kp::error(KP_ERR_ARG, w + ": n_bins must be in [1, 2^30)");
+    if (!offsets || (n > 0 && (!keys || !ws)))
+        throw kp::error(KP_ERR_ARG, w + ": null argument");
+    const bin_layout L = bin_ws((uint64_t)n);
+    if (n > 0 && ws_bytes < L.total)
+        throw kp::error(KP_ERR_ARG, w + ": workspace too small (see "
+                                        "kp_compact_workspace_bytes)");
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for((uint64_t)n_bins + 1)), dim3(kThreads),
+                           0, s, offsets, (uint64_t)n_bins + 1, (int64_t)0);
+        KP_HIPC(hipGetLastError());
+        return;
+    }
+    char *w8 = (char *)ws;
+    uint32_t *ka = (uint32_t *)(w8 + L.ka), *kb = (uint32_t *)(w8 + L.kb);
+    uint32_t *ia = (uint32_t *)(w8 + L.ia), *ib = (uint32_t *)(w8 + L.ib);
+    uint32_t *hist = (uint32_t *)(w8 + L.hist);
+    const uint64_t nt = ((uint64_t)n + kTile - 1) / kTile;
+    scan_ws sw{(uint32_t *)(w8 + L.sums), scan_chunks(256 * nt)};
+    // key n_bins ("stays") needs the bits of n_bins itself
+    int bits = 1;
+    while (bits < 31 && ((uint64_t)n_bins >> bits) != 0)
+        ++bits;
+    const int passes = (bits + 7) / 8;
+    const uint32_t *kin = nullptr, *iin = nullptr;
+    uint32_t *kout = ka, *iout = ia;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, (const uint32_t *)nullptr, (uint64_t)n, n_bins, shift,
+                               hist, nt, (const unsigned long long *)nullptr);
+        else
+            hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, kin, (uint64_t)n, n_bins, shift, hist, nt,
+                               (const unsigned long long *)nullptr);
+        exscan(hist, 256 * nt, sw, nullptr, s);
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                               kout, iout, (uint64_t)n, n_bins, shift, hist, nt,
+                               (const unsigned long long *)nullptr);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nt), dim3(kThreads), 0, s,
+                               keys, kin, iin, kout, iout, (uint64_t)n, n_bins, shift, hist,
+                               nt, (const unsigned long long *)nullptr);
+        kin = kout;
+        iin = iout;
+        kout = (kout == ka) ? kb : ka;
+        iout = (iout == ia) ? ib : ia;
+    }
+    hipLaunchKernelGGL(k_bin_offsets, dim3(grid_for((uint64_t)n + 1)), dim3(kThreads), 0, s,
+                       kin, (uint64_t)n, n_bins, offsets);
+    if (order || slot)
+        hipLaunchKernelGGL(k_bin_slots, dim3(grid_for((uint64_t)n)), dim3(kThreads), 0, s, iin,
+                           (uint64_t)n, offsets, n_bins, order, slot);
+    KP_HIPC(hipGetLastError());
+}
+
+} // namespace
+
+// Workspace of kp_bin_by_key_static: a fixed device array, one copy per GPU,
+// sized for KP_BIN_STATIC_MAX records (every part of it is written before it
+// is read, so nothing needs clearing between calls).  Host code never uses
+// its name as a device address: static_scratch() gives this device's copy,
+// looked up once per device with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[bin_ws(KP_BIN_STATIC_MAX).total];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+void *static_scratch()
+{
+    int dev = 0;
+    KP_HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        throw kp::error(KP_ERR_UNSUPPORTED, "kp_bin_by_key_static: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        KP_HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            throw kp::error(KP_ERR_HIP, "kp_bin_by_key_static: no device address for the "
+                                        "static workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 } // namespace
@@ -569,67 +671,23 @@ kp_status kp_bin_by_key(const int32_t *keys, int64_t n, int32_t n_bins, int64_t 
                         void *stream_)
 {
     return kp::guard([&] {
-        if (n < 0 || n >= (1ll << 31))
-            throw kp::error(KP_ERR_ARG, "kp_bin_by_key: n must be in [0, 2^31)");
-        if (n_bins < 1 || n_bins >= (1 << 30))
-            throw kp::error(KP_ERR_ARG, "kp_bin_by_key: n_bins must be in [1, 2^30)");
-        if (!offsets || (n > 0 && (!keys || !ws)))
-            throw kp::error(KP_ERR_ARG, "kp_bin_by_key: null argument");
-        const bin_layout L = bin_ws((uint64_t)n);
-        if (n > 0 && ws_bytes < L.total)
-            throw kp::error(KP_ERR_ARG, "kp_bin_by_key: workspace too small (see "
-                                        "kp_compact_workspace_bytes)");
-        hipStream_t s = (hipStream_t)stream_;
-        if (n == 0) {
-            hipLaunchKernelGGL(k_fill_i64, dim3(grid_for((uint64_t)n_bins + 1)), dim3(kThreads),
-                               0, s, offsets, (uint64_t)n_bins + 1, (int64_t)0);
-            KP_HIPC(hipGetLastError());
-            return;
-        }
-        char *w = (char *)ws;
-        uint32_t *ka = (uint32_t *)(w + L.ka), *kb = (uint32_t *)(w + L.kb);
-        uint32_t *ia = (uint32_t *)(w + L.ia), *ib = (uint32_t *)(w + L.ib);
-        uint32_t *hist = (uint32_t *)(w + L.hist);
-        const uint64_t nt = ((uint64_t)n + kTile - 1) / kTile;
-        scan_ws sw{(uint32_t *)(w + L.sums), scan_chunks(256 * nt)};
-        // key n_bins ("stays") needs the bits of n_bins itself
-        int bits = 1;
-        while (bits < 31 && ((uint64_t)n_bins >> bits) != 0)
-            ++bits;
-        const int passes = (bits + 7) / 8;
-        const uint32_t *kin = nullptr, *iin = nullptr;
-        uint32_t *kout = ka, *iout = ia;
-        for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            if (p == 0)
-                hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nt), dim3(kThreads), 0,
-                                   s, keys, (const uint32_t *)nullptr, (uint64_t)n, n_bins,
-                                   shift, hist, nt, (const unsigned long long *)nullptr);
-            else
-                hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nt), dim3(kThreads), 0,
-                                   s, keys, kin, (uint64_t)n, n_bins, shift, hist, nt,
-                                   (const unsigned long long *)nullptr);
-            exscan(hist, 256 * nt, sw, nullptr, s);
-            if (p == 0)
-                hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nt), dim3(kThreads), 0,
-                                   s, keys, (const uint32_t *)nullptr,
-                                   (const uint32_t *)nullptr, kout, iout, (uint64_t)n, n_bins,
-                                   shift, hist, nt, (const unsigned long long *)nullptr);
-            else
-                hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nt), dim3(kThreads),
-                                   0, s, keys, kin, iin, kout, iout, (uint64_t)n, n_bins,
-                                   shift, hist, nt, (const unsigned long long *)nullptr);
-            kin = kout;
-            iin = iout;
-            kout = (kout == ka) ? kb : ka;
-            iout = (iout == ia) ? ib : ia;
-        }
-        hipLaunchKernelGGL(k_bin_offsets, dim3(grid_for((uint64_t)n + 1)), dim3(kThreads), 0, s,
-                           kin, (uint64_t)n, n_bins, offsets);
-        if (order || slot)
-            hipLaunchKernelGGL(k_bin_slots, dim3(grid_for((uint64_t)n)), dim3(kThreads), 0, s,
-                               iin, (uint64_t)n, offsets, n_bins, order, slot);
-        KP_HIPC(hipGetLastError());
+        bin_by_key_run("kp_bin_by_key", keys, n, n_bins, offsets, order, slot, ws, ws_bytes,
+                       (hipStream_t)stream_);
+    });
+}
+
+kp_status kp_bin_by_key_static(const int32_t *keys, int64_t n, int32_t n_bins,
+                               int64_t *offsets, int32_t *order, int32_t *slot, void *stream_)
+{
+    return kp::guard([&] {
+        if (n < 0 || n > (int64_t)KP_BIN_STATIC_MAX)
+            throw kp::error(KP_ERR_ARG, "kp_bin_by_key_static: n must be in [0, " +
+                                            std::to_string(KP_BIN_STATIC_MAX) +
+                                            "] (the static workspace's size); larger runs "
+                                            "need kp_bin_by_key with a workspace");
+        void *ws = static_scratch();
+        bin_by_key_run("kp_bin_by_key_static", keys, n, n_bins, offsets, order, slot, ws,
+                       sizeof(scratch), (hipStream_t)stream_);
     });
 }
 
