@@ -11,7 +11,7 @@
  * Reference interfaces replaced (paths relative to the reference src/):
  *   mesh tree + face-neighbour table ... mesh/tree/tree.cpp:177-218,
  *                                        modules/particle/types/proxy.h:22-62,
- *                                        modules/particle/types/block_map.h:225-275
+ *                                        modules/particle/types/block_map.h:53-103
  *   block geometry ..................... mesh/meshgen/meshgen.cpp:286-351,378-432
  *   per-cell radiation storage ......... modules/particle/radiation/rad_data.cpp:13-25
  *   one particle step (generate +
@@ -144,7 +144,7 @@ kp_status kp_mesh_locate(const kp_mesh *mesh, const double x[3],
 typedef struct kp_engine kp_engine;
 
 /* Copies the block table to the device (block_map_t::update,
- * block_map.h:225-275).  The engine is bound to `device`. */
+ * block_map.h:53-103).  The engine is bound to `device`. */
 kp_status kp_engine_create(const kp_mesh_desc *mesh, int32_t device,
                            kp_engine **out);
 void kp_engine_destroy(kp_engine *eng);
