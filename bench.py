@@ -188,6 +188,7 @@ def batched(steps, scale):
         t0 = time.perf_counter()
         sched = eng.autotune(w.source, run, sample_packets=10_000_000)
         res["autotune_s"] += time.perf_counter() - t0
+        progress(f"batched: precision {prec} tuned in {time.perf_counter() - t0:.1f} s")
         if order is not None:
             sched = {**sched, "coherent": order}
             eng.set_schedule(**sched)
@@ -203,6 +204,7 @@ def batched(steps, scale):
             g.launch()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        progress(f"batched: precision {prec}, order {order}, {1e3 * wall / steps:.0f} ms per step")
         cnt = out["counters"].cpu().numpy().astype(np.float64)
         img = out["image"].cpu().numpy()[:, 0]
         del g

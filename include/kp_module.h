@@ -27,7 +27,8 @@
  *            allowed), kappa_scale (n_hv values, 1), source (3 reals,
  *            centre of the domain), weight (1), incl (degrees, none),
  *            image_n (128), image_half (0.5), precision (f64|f32),
- *            rr_weight (0.01), rr_survive (0.1), schedule (persistent|history),
+ *            rr_weight (0.01), rr_survive (0.1), schedule (persistent|history|
+ *            block_queue),
  *            order (slot|direction: packets of a launch in emission-direction
  *            order, kp_schedule::coherent; the engine's default if absent)
  *   On a spherical mesh the photon streams (photon_sph.h): kappa_scale

@@ -236,8 +236,23 @@ typedef struct kp_tallies {
  *   thread (photon.h:104-125).
  * KP_SCHED_PERSISTENT: CUs*blocks_per_cu resident blocks; each lane runs a
  *   one-crossing-per-iteration state machine and refills from a wave slot
- *   range of `chunk` packets claimed with one atomic.  Same histories. */
-enum { KP_SCHED_HISTORY = 0, KP_SCHED_PERSISTENT = 1 };
+ *   range of `chunk` packets claimed with one atomic.  Same histories.
+ * KP_SCHED_BLOCKQ: block-bucketed packet queues (the per-block pools of
+ *   types/particle_pool.h:191-248 and the per-destination compaction of
+ *   boundary/comm.h:118-127, with blocks as destinations).  Bands from
+ *   queue_from_band on run as passes: the packets' records (144 B FP64,
+ *   96 B FP32) are sorted by (band, block); a workgroup takes queue_chunk
+ *   sorted packets, stages each of their blocks' kappa_rho and an FP64 J tile
+ *   in LDS, and marches every packet until it leaves the block or ends
+ *   (interactions and peel-offs included), then adds the tile to J with one
+ *   atomic per touched cell; packets that cross into another block are
+ *   compacted into the next pass's records by wave ballot and prefix rank.
+ *   After queue_passes passes the packets still alive finish one thread
+ *   each.  Bands before queue_from_band run on the persistent kernel in the
+ *   same launch.  Same histories; needs n_cell^3 <= 4096 and fewer than 2^31
+ *   queued packets per launch.  Record memory: 2 x record size + 16 B per
+ *   queued packet, kept by the engine. */
+enum { KP_SCHED_HISTORY = 0, KP_SCHED_PERSISTENT = 1, KP_SCHED_BLOCKQ = 2 };
 typedef struct kp_schedule {
     int32_t kind;
     int32_t block_threads;  /* 64, 128 or 256 (the reference's [particle] n_th) */
@@ -300,6 +315,13 @@ typedef struct kp_schedule {
     /* Largest chunk of the one-pass replay, in packets (0 = as large as the
      * queue memory allows; otherwise at least 1024). */
     int32_t adjoint_chunk;
+    /* KP_SCHED_BLOCKQ: first band of the block queue (0 = every band; the
+     * bands before it run on the persistent kernel), passes before the
+     * packets still alive finish one thread each (0 = 32), sorted packets
+     * per workgroup task (0 = 1024). */
+    int32_t queue_from_band;
+    int32_t queue_passes;
+    int32_t queue_chunk;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

@@ -92,7 +92,7 @@ class Run(C.Structure):
     ]
 
 
-KP_SCHED_HISTORY, KP_SCHED_PERSISTENT = 0, 1
+KP_SCHED_HISTORY, KP_SCHED_PERSISTENT, KP_SCHED_BLOCKQ = 0, 1, 2
 
 
 class Schedule(C.Structure):
@@ -109,6 +109,9 @@ class Schedule(C.Structure):
         ("peel_queue", C.c_int32),
         ("vertex_slots", C.c_int32),
         ("adjoint_chunk", C.c_int32),
+        ("queue_from_band", C.c_int32),
+        ("queue_passes", C.c_int32),
+        ("queue_chunk", C.c_int32),
     ]
 
 

@@ -1,0 +1,376 @@
+// kp_blockq.h -- block-bucketed packet queues (schedule KP_SCHED_BLOCKQ).
+//
+// The reference moves packets between per-block pools: a packet that leaves
+// its block is marked for relocation and compacted out of the pool
+// (types/particle_pool.h:191-248), one that leaves the rank is registered and
+// gathered per destination (boundary/comm.h:118-127, 226-239).  Here the same
+// idea runs per block, on one GPU:
+//   - every packet lives in a record (AoS, 144 B in FP64, 96 B in FP32:
+//     position, direction, Stokes frame and vector, optical depth left, block,
+//     cell, counts), keyed by (band, block);
+//   - a pass sorts the records by key (stable LSD radix sort) and gives each
+//     workgroup a run of sorted positions; per block of that run the
+//     workgroup stages the block's kappa_rho (n_cell^3 floats) and an FP64 J
+//     tile in LDS, marches each of its packets until the packet leaves the
+//     block or ends (interactions and peel-offs included; peel-off rays read
+//     kappa_rho from HBM), deposits J into the LDS tile, and flushes the tile
+//     once with one FP64 atomic per touched cell;
+//   - a packet that crosses into another block is written to the next pass's
+//     records at a slot claimed once per wave (ballot / prefix rank), with
+//     its new key; one that ends is counted and its end state written.
+// After queue_passes passes the packets still alive run to their end in one
+// thread each, with HBM kappa_rho and global J atomics.
+// Every packet performs the per-packet arithmetic of its precision in the
+// same order as the other schedules (FP64 histories are bit-identical to the
+// CPU oracle); only the J summation order differs.
+#pragma once
+
+#include "kp_persist.h"
+
+namespace kp {
+namespace bq {
+
+// Largest block (n_cell^3 cells) whose kappa_rho and J tile fit the LDS of a
+// workgroup: 16^3 cells, 48 KB.
+constexpr int kMaxCells = 4096;
+
+template <class R> struct rec_t;
+template <> struct __align__(16) rec_t<double> {
+    double x[3], d[3], e1[3], s[3]; // position (global), direction, frame, Stokes I Q U
+    double tau;                      // optical depth left of the free flight
+    uint32_t slot;                   // launch slot (band and packet id)
+    int32_t ib;                      // block
+    uint32_t cell;                   // i0 | i1 << 8 | i2 << 16
+    int32_t steps;                   // march steps left
+    uint32_t events, ncross, npeel, pad;
+};
+template <> struct __align__(16) rec_t<float> {
+    float x[3], d[3], e1[3], s[3];   // position (block-local cell units), ...
+    double tau;
+    uint32_t slot;
+    int32_t ib;
+    uint32_t cell;
+    int32_t steps;
+    uint32_t events, ncross, npeel, pad;
+};
+static_assert(sizeof(rec_t<double>) == 144 && sizeof(rec_t<float>) == 96, "record sizes");
+
+// Launch parameters of the queue kernels (besides kparams).
+struct qparams {
+    const void *in;                   // records of this pass
+    const uint32_t *in_key;           // their keys
+    const int32_t *order;             // sorted position -> record
+    const uint32_t *seg;              // [n_keys + 1] first sorted position of each key
+    const unsigned long long *n_in;   // records of this pass
+    void *out;                        // records that crossed into another block
+    uint32_t *out_key;
+    unsigned long long *n_out;        // claimed out records (zeroed before the pass)
+    uint32_t n_keys;                  // queue bands x blocks
+    uint32_t chunk;                   // sorted positions per workgroup task
+    int first_band;                   // key = (band - first_band) * n_blocks + block
+    int n_blocks;
+    uint64_t first_slot;              // first launch slot of the queue bands
+    uint64_t n_slots;                 // slots of the queue bands
+    uint32_t *emit_np;                // [n_bands] peel-off crossings of the emission peel-offs
+};
+
+// A packet between march steps.
+template <class Pol> struct qlane {
+    using real = typename Pol::real;
+    typename Pol::ray r;
+    real e1[3];
+    real I, Q, U;
+    real ks;
+    double tau;
+    uint64_t slot, pid;
+    int band, events, steps, status;
+    uint32_t ncross, npeel;
+};
+
+template <class Pol> __device__ __forceinline__ void unpack_cell(typename Pol::ray &r, uint32_t c)
+{
+    r.i[0] = (int)(c & 255u);
+    r.i[1] = (int)((c >> 8) & 255u);
+    r.i[2] = (int)((c >> 16) & 255u);
+}
+
+template <class Pol>
+__device__ __forceinline__ void load(const kparams &P, const rec_t<typename Pol::real> &q,
+                                     qlane<Pol> &L)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        L.r.x[a] = q.x[a];
+        L.r.d[a] = q.d[a];
+        L.e1[a] = q.e1[a];
+    }
+    L.I = q.s[0];
+    L.Q = q.s[1];
+    L.U = q.s[2];
+    L.tau = q.tau;
+    L.slot = q.slot;
+    L.r.ib = q.ib;
+    unpack_cell<Pol>(L.r, q.cell);
+    L.steps = q.steps;
+    L.events = (int)q.events;
+    L.ncross = q.ncross;
+    L.npeel = q.npeel;
+    L.band = band_of(P, L.slot);
+    L.pid = P.packet_offset + (L.slot - P.band_start[L.band]);
+    L.ks = (typename Pol::real)P.bands[L.band].kappa_scale;
+    Pol::relocate(P, L.r); // FP64: face bounds; FP32: per-ray scales of the block
+}
+
+template <class Pol>
+__device__ __forceinline__ void store(const qlane<Pol> &L, rec_t<typename Pol::real> &q)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        q.x[a] = L.r.x[a];
+        q.d[a] = L.r.d[a];
+        q.e1[a] = L.e1[a];
+    }
+    q.s[0] = L.I;
+    q.s[1] = L.Q;
+    q.s[2] = L.U;
+    q.tau = L.tau;
+    q.slot = (uint32_t)L.slot;
+    q.ib = L.r.ib;
+    q.cell = (uint32_t)L.r.i[0] | ((uint32_t)L.r.i[1] << 8) | ((uint32_t)L.r.i[2] << 16);
+    q.steps = L.steps;
+    q.events = (uint32_t)L.events;
+    q.ncross = L.ncross;
+    q.npeel = L.npeel;
+    q.pad = 0;
+}
+
+// Interaction at the packet's point (in cell gc): Philox words of the event,
+// absorption weighting, the peel-offs toward every observer (rays marched to
+// the domain edge through HBM kappa_rho), the scattering and Russian
+// roulette.  The operations and their order are those of the other
+// schedules.  Returns false when roulette ends the packet.
+template <class Pol> __device__ bool interact(const kparams &P, qlane<Pol> &L, int64_t gc)
+{
+    using real = typename Pol::real;
+    L.events += 1;
+    uint32_t u[4] = {(uint32_t)L.pid, (uint32_t)(L.pid >> 32), (uint32_t)L.events, 0u};
+    kp_philox4x32_10(u, P.seed, (uint32_t)L.band);
+    const dband &bd = P.bands[L.band];
+    uint64_t np = 0;
+    if constexpr (sizeof(real) == 8) {
+        const double alb = (bd.flags & KP_BAND_CELL_ALBEDO) ? (double)P.albedo[gc] : bd.albedo;
+        const double gg = (bd.flags & KP_BAND_CELL_G) ? (P.g ? (double)P.g[gc] : 0.0) : bd.g;
+        L.I = L.I * alb;
+        L.Q = L.Q * alb;
+        L.U = L.U * alb;
+        double e2[3];
+        f64::cross3(L.r.d, L.e1, e2);
+        for (int o = 0; o < P.n_incl; ++o) {
+            double Ip, Qi, Ui;
+            f64::peel_stokes(P.model, gg, P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, P.no[o],
+                             P.ev[o], Ip, Qi, Ui);
+            const double tau = f64::tau_to_edge(P, L.r, P.no[o], L.ks, np);
+            const double att = kp_exp(-tau) * KP_INV_4PI;
+            f64::deposit(P, L.band, o, L.r.x, Ip * att, Qi * att, Ui * att);
+        }
+        const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
+                                                  : kp_hg_mu(gg, kp_u01(u[0]));
+        f64::scatter_stokes(P.model, P.pmax, mu, kp_u01(u[1]), L.r.d, L.e1, e2, L.I, L.Q, L.U);
+        L.tau = -kp_log(kp_u01(u[2]));
+        L.npeel += (uint32_t)np;
+        if (L.I < P.rr_w * P.w0) {
+            if (kp_u01(u[3]) < P.rr_s) {
+                L.I = L.I / P.rr_s;
+                L.Q = L.Q / P.rr_s;
+                L.U = L.U / P.rr_s;
+            } else {
+                return false;
+            }
+        }
+    } else {
+        const float alb = (bd.flags & KP_BAND_CELL_ALBEDO) ? P.albedo[gc] : (float)bd.albedo;
+        const float gg = (bd.flags & KP_BAND_CELL_G) ? (P.g ? P.g[gc] : 0.0f) : (float)bd.g;
+        L.I *= alb;
+        L.Q *= alb;
+        L.U *= alb;
+        float e2[3];
+        f32::cross3(L.r.d, L.e1, e2);
+        if (P.n_incl > 0) {
+            float gx[3];
+            f32::global_pos(P, L.r, gx);
+            for (int o = 0; o < P.n_incl; ++o) {
+                float no[3], ev[3];
+                load3(P.no[o], no);
+                load3(P.ev[o], ev);
+                float Ip, Qi, Ui;
+                f32::peel_stokes(P.model, gg, (float)P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, no,
+                                 ev, Ip, Qi, Ui);
+                const double tau = f32::tau_to_edge(P, L.r, no, L.ks, np);
+                const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
+                f32::deposit(P, L.band, o, gx, Ip * att, Qi * att, Ui * att);
+            }
+        }
+        const float xi0 = f32::u01(u[0]);
+        const float mu = (P.model == KP_THOMSON) ? f32::thomson_mu(xi0) : f32::hg_mu(gg, xi0);
+        f32::scatter_stokes(P.model, (float)P.pmax, mu, f32::u01(u[1]), L.r.d, L.e1, e2, L.I,
+                            L.Q, L.U);
+        Pol::set_dir(P, L.r, L.r.d);
+        L.tau = -(double)f32::fast_log(f32::u01(u[2]));
+        L.npeel += (uint32_t)np;
+        if (L.I < (float)P.rr_w * (float)P.w0) {
+            if (f32::u01(u[3]) < (float)P.rr_s) {
+                const float inv = (float)(1.0 / P.rr_s);
+                L.I *= inv;
+                L.Q *= inv;
+                L.U *= inv;
+            } else {
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+enum : int { MARCH_END = 0, MARCH_OUT = 1 };
+
+// The packet's march from its current point.  LOCAL: kappa_rho from the LDS
+// tile lk of the packet's block and J into the LDS tile lj; returns
+// MARCH_OUT as soon as the ray has crossed into another block (the lane then
+// holds it in the new block).  Otherwise kappa_rho from HBM and J by global
+// atomics, to the packet's end.  MARCH_END: L.status holds the end status.
+template <class Pol, bool LOCAL>
+__device__ int march(const kparams &P, qlane<Pol> &L, const float *lk, double *lj)
+{
+    using real = typename Pol::real;
+    for (;;) {
+        int64_t gc = 0;
+        for (;;) {
+            if (L.steps <= 0) {
+                L.status = KP_END_STEPS;
+                return MARCH_END;
+            }
+            real l;
+            const int ap = Pol::dl(L.r, l);
+            if (ap < 0) {
+                L.status = KP_END_LOST;
+                return MARCH_END;
+            }
+            const int cell = Pol::cell(P, L.r);
+            gc = (int64_t)L.r.ib * P.n3 + cell;
+            const float kraw = LOCAL ? lk[cell] : P.kappa[gc];
+            const real kap = (real)kraw * L.ks;
+            const real dtau = kap * l;
+            L.ncross += 1;
+            L.steps -= 1;
+            bool hit;
+            real seg;
+            if constexpr (sizeof(real) == 8) {
+                hit = kap > 0.0 && dtau >= L.tau;
+                seg = l;
+                if (hit) {
+                    const double li = L.tau / kap;
+                    seg = li > l ? l : li;
+                }
+            } else {
+                hit = kap > 0.0f && (double)dtau >= L.tau;
+                seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
+            }
+            if (P.J) {
+                double v;
+                if constexpr (sizeof(real) == 8)
+                    v = L.I * (seg / P.vol0[L.r.ib]);
+                else
+                    v = (double)(L.I * seg * L.r.iv);
+                if constexpr (LOCAL)
+                    atomicAdd(&lj[cell], v);
+                else
+                    atomic_add_f64(j_at(P, (uint64_t)gc, L.band), v);
+            }
+            if (hit) {
+                Pol::advance(L.r, seg);
+                break;
+            }
+            L.tau = L.tau - (double)dtau;
+            int st = Pol::cross_step(P, L.r, ap, l);
+            if (st == f32::ST_IN)
+                continue;
+            st = Pol::change_block(P, L.r, ap, st == f32::ST_BLOCK_HI);
+            if (st == f32::ST_ESCAPED) {
+                L.status = KP_END_ESCAPED;
+                return MARCH_END;
+            }
+            if (st != f32::ST_IN) {
+                L.status = KP_END_LOST;
+                return MARCH_END;
+            }
+            if constexpr (LOCAL)
+                return MARCH_OUT;
+        }
+        if (!interact<Pol>(P, L, gc)) {
+            L.status = KP_END_RR;
+            return MARCH_END;
+        }
+    }
+}
+
+// Counters and end state of a packet that ended.
+template <class Pol>
+__device__ __forceinline__ void end_packet(const kparams &P, const qlane<Pol> &L, lane_counters &C)
+{
+    C.cross += L.ncross;
+    C.peel += L.npeel;
+    C.events += (uint32_t)L.events;
+    C.packets += 1;
+    if (L.status == KP_END_ESCAPED)
+        C.escaped += 1;
+    else if (L.status == KP_END_RR)
+        C.rr += 1;
+    else if (L.status == KP_END_STEPS)
+        C.steps += 1;
+    else
+        C.lost += 1;
+    if (P.end_state) {
+        double *es = P.end_state + L.slot * KP_ES_STRIDE;
+        double g[3];
+        Pol::gpos_d(P, L.r, g);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            es[KP_ES_X + a] = g[a];
+            es[KP_ES_D + a] = (double)L.r.d[a];
+        }
+        es[KP_ES_I] = (double)L.I;
+        es[KP_ES_Q] = (double)L.Q;
+        es[KP_ES_U] = (double)L.U;
+        es[KP_ES_EVENTS] = (double)L.events;
+        es[KP_ES_CROSS] = (double)L.ncross;
+        es[KP_ES_STATUS] = (double)L.status;
+        es[KP_ES_IB] = (double)L.r.ib;
+        es[KP_ES_CELL] = (double)Pol::cell(P, L.r);
+        es[14] = (double)L.npeel;
+        es[15] = 0.0;
+    }
+}
+
+// Record of the next pass for each calling lane that crossed into another
+// block: one slot claim per wave (ballot, then the lane's rank among the
+// writers), the record and its key written.  Called by the writing lanes
+// together.
+template <class Pol>
+__device__ __forceinline__ void push_out(const kparams &P, const qparams &Q, const qlane<Pol> &L)
+{
+    using real = typename Pol::real;
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t rank = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd(Q.n_out, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const uint64_t k = base + rank;
+    store<Pol>(L, static_cast<rec_t<real> *>(Q.out)[k]);
+    Q.out_key[k] = (uint32_t)(L.band - Q.first_band) * (uint32_t)Q.n_blocks + (uint32_t)L.r.ib;
+}
+
+} // namespace bq
+} // namespace kp

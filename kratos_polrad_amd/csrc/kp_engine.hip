@@ -10,6 +10,7 @@
 // rad_t::flx (radiation.h:24-35).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -22,6 +23,7 @@
 #include "kp_error.h"
 #include "kp_adjoint.h"
 #include "kp_persist.h"
+#include "kp_blockq.h"
 #include "kp_sph.h"
 
 #define KP_HIP(call)                                                                        \
@@ -43,6 +45,16 @@ struct fwd_bufs {
     int32_t *keys = nullptr, *order = nullptr;
     void *ws = nullptr;
     size_t ord_cap = 0, ws_cap = 0; // elements / bytes
+    // block queue: two record sets and their keys, the sort permutation,
+    // sorted keys, per-key segment starts, sort workspace, record counters,
+    // emission peel-off crossings per band
+    void *bq_rec[2] = {nullptr, nullptr};
+    uint32_t *bq_key[2] = {nullptr, nullptr};
+    int32_t *bq_order = nullptr;
+    uint32_t *bq_skey = nullptr, *bq_seg = nullptr, *bq_np = nullptr;
+    void *bq_ws = nullptr;
+    unsigned long long *bq_cnt = nullptr;
+    size_t bq_cap = 0, bq_rec_bytes = 0, bq_seg_cap = 0, bq_np_cap = 0, bq_ws_cap = 0;
 };
 
 struct kp_engine {
@@ -392,6 +404,196 @@ persist_fn defer_variant(int precision, int wps)
 }
 
 
+// ---------------------------------------------------------------- block queue
+// Emission peel-offs of the queue bands: every packet of a band starts at the
+// source with the same Stokes vector, so its emission peel-off rays are one
+// ray per observer.  One thread per band marches them, adds n_packets times
+// the packet's deposit at the source pixel and records the rays' crossings
+// (each packet's peel crossing count starts with them).
+template <class Pol>
+__global__ void __launch_bounds__(64) k_bq_emit_peel(kp::kparams P, kp::bq::qparams Q)
+{
+    using real = typename Pol::real;
+    const int b = Q.first_band + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= P.n_bands)
+        return;
+    const uint64_t n = P.band_start[b + 1] - P.band_start[b];
+    uint64_t np = 0;
+    if (n > 0 && P.n_incl > 0) {
+        typename Pol::ray r;
+        r.d[0] = 0;
+        r.d[1] = 0;
+        r.d[2] = 1;
+        Pol::at_source(P, r);
+        const real ks = (real)P.bands[b].kappa_scale;
+        for (int o = 0; o < P.n_incl; ++o) {
+            double v;
+            long px;
+            if constexpr (sizeof(real) == 8) {
+                const double tau = kp::f64::tau_to_edge(P, r, P.no[o], ks, np);
+                v = P.w0 * (kp_exp(-tau) * KP_INV_4PI);
+                px = kp::f64::pixel(P, b, o, r.x);
+            } else {
+                float no[3];
+                kp::load3(P.no[o], no);
+                const double tau = kp::f32::tau_to_edge(P, r, no, ks, np);
+                const float att = kp::f32::fast_exp(-(float)tau) * kp::f32::kInv4Pi;
+                v = (double)((float)P.w0 * att);
+                float gx[3];
+                kp::f32::global_pos(P, r, gx);
+                px = kp::f32::pixel(P, b, o, gx);
+            }
+            if (P.image && px >= 0)
+                kp::atomic_add_f64(kp::img_base(P) + 3 * px, (double)n * v);
+        }
+    }
+    Q.emit_np[b] = (uint32_t)np;
+}
+
+// Records of the queue bands' packets at emission, all in the source block.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_bq_generate(kp::kparams P, kp::bq::qparams Q)
+{
+    using real = typename Pol::real;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < Q.n_slots;
+         k += stride) {
+        kp::bq::qlane<Pol> L;
+        L.slot = Q.first_slot + k;
+        L.band = kp::band_of(P, L.slot);
+        L.pid = P.packet_offset + (L.slot - P.band_start[L.band]);
+        uint32_t rn[4] = {(uint32_t)L.pid, (uint32_t)(L.pid >> 32), 0u, 0u};
+        kp_philox4x32_10(rn, P.seed, (uint32_t)L.band);
+        if constexpr (sizeof(real) == 8)
+            kp::f64::emit(rn, L.r.d, L.e1, L.tau);
+        else
+            kp::f32::emit(rn, L.r.d, L.e1, L.tau);
+        L.I = (real)P.w0;
+        L.Q = 0;
+        L.U = 0;
+        Pol::at_source(P, L.r);
+        L.steps = P.step_max;
+        L.events = 0;
+        L.ncross = 0;
+        L.npeel = Q.emit_np[L.band];
+        kp::bq::store<Pol>(L, static_cast<kp::bq::rec_t<real> *>(Q.out)[k]);
+        Q.out_key[k] = (uint32_t)(L.band - Q.first_band) * (uint32_t)Q.n_blocks + (uint32_t)P.src_ib;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *Q.n_out = Q.n_slots;
+}
+
+// skey[p] = key of the p-th sorted record
+__global__ void __launch_bounds__(256) k_bq_skey(const uint32_t *key, const int32_t *order,
+                                                 const unsigned long long *n_p, uint32_t *skey)
+{
+    const uint64_t n = *n_p;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        skey[p] = key[order[p]];
+}
+
+// seg[k] = first sorted position whose key is >= k, for k in [0, n_keys]
+__global__ void __launch_bounds__(256) k_bq_seg(const uint32_t *skey,
+                                                const unsigned long long *n_p, uint32_t n_keys,
+                                                uint32_t *seg)
+{
+    const uint64_t n = *n_p;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n_keys; k += stride) {
+        uint64_t lo = 0, hi = n; // lower bound of k in skey[0, n)
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (skey[mid] < (uint32_t)k)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        seg[k] = (uint32_t)lo;
+    }
+}
+
+// One pass of the block queue: tasks of `chunk` sorted records; per block of
+// a task the block's kappa_rho and a J tile in LDS, each packet marched to
+// the block's edge or its end, the tile added to J.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams Q)
+{
+    using real = typename Pol::real;
+    double *lj = kp::kp_lds;
+    float *lk = reinterpret_cast<float *>(kp::kp_lds + P.n3);
+    __shared__ uint32_t s_key;
+    __shared__ uint64_t s_end;
+    kp::lane_counters C;
+    const uint64_t n = *Q.n_in;
+    const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
+    for (uint64_t t = blockIdx.x; t * Q.chunk < n; t += gridDim.x) {
+        uint64_t p = t * Q.chunk;
+        const uint64_t p1 = (p + Q.chunk < n) ? p + Q.chunk : n;
+        while (p < p1) {
+            if (threadIdx.x == 0) {
+                const uint32_t key = Q.in_key[Q.order[p]];
+                uint64_t e = Q.seg[key + 1];
+                if (e > p1)
+                    e = p1;
+                if (e <= p) // progress whatever the table holds
+                    e = p1;
+                s_key = key;
+                s_end = e;
+            }
+            __syncthreads();
+            const uint32_t key = s_key;
+            const uint64_t e = s_end;
+            const int ib = (int)(key % (uint32_t)Q.n_blocks);
+            const int band = Q.first_band + (int)(key / (uint32_t)Q.n_blocks);
+            const float *kb = P.kappa + (uint64_t)ib * P.n3;
+            for (int c = threadIdx.x; c < P.n3; c += blockDim.x) {
+                lk[c] = kb[c];
+                lj[c] = 0.0;
+            }
+            __syncthreads();
+            for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
+                kp::bq::qlane<Pol> L;
+                kp::bq::load<Pol>(P, in[Q.order[q]], L);
+                if (kp::bq::march<Pol, true>(P, L, lk, lj) == kp::bq::MARCH_OUT)
+                    kp::bq::push_out<Pol>(P, Q, L);
+                else
+                    kp::bq::end_packet<Pol>(P, L, C);
+            }
+            __syncthreads();
+            if (P.J) {
+                const uint64_t g0 = (uint64_t)ib * P.n3;
+                for (int c = threadIdx.x; c < P.n3; c += blockDim.x) {
+                    const double v = lj[c];
+                    if (v != 0.0)
+                        kp::atomic_add_f64(kp::j_at(P, g0 + c, band), v);
+                }
+            }
+            __syncthreads();
+            p = e;
+        }
+    }
+    flush_counters(P, C);
+}
+
+// Packets still alive after the last pass: each runs to its end in one thread.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparams Q)
+{
+    using real = typename Pol::real;
+    kp::lane_counters C;
+    const uint64_t n = *Q.n_in;
+    const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        kp::bq::qlane<Pol> L;
+        kp::bq::load<Pol>(P, in[k], L);
+        (void)kp::bq::march<Pol, false>(P, L, nullptr, nullptr);
+        kp::bq::end_packet<Pol>(P, L, C);
+    }
+    flush_counters(P, C);
+}
+
 // Sort key of every queued peel-off ray: observer, then the Morton code of
 // its start point's 2^9 x 2^9 image-plane cell, then its depth along the line
 // of sight (2^9 steps); unused slots sort last.
@@ -687,6 +889,17 @@ void free_bufs(fwd_bufs &b)
     dfree(b.order);
     dfree(b.ws);
     b.ord_cap = b.ws_cap = 0;
+    for (int k = 0; k < 2; ++k) {
+        dfree(b.bq_rec[k]);
+        dfree(b.bq_key[k]);
+    }
+    dfree(b.bq_order);
+    dfree(b.bq_skey);
+    dfree(b.bq_seg);
+    dfree(b.bq_np);
+    dfree(b.bq_ws);
+    dfree(b.bq_cnt);
+    b.bq_cap = b.bq_rec_bytes = b.bq_seg_cap = b.bq_np_cap = b.bq_ws_cap = 0;
 }
 
 void free_all(kp_engine *e)
@@ -963,7 +1176,58 @@ struct fwd_plan {
     bool j_band_major = false; // J accumulated band-major in replicas, folded by k_fold_t
     bool coherent = false;     // packets marched in emission-direction order
     int dir_bits = 0, key_bits = 0;
+    // block queue (KP_SCHED_BLOCKQ): bands [bq_first_band, n_bands), their
+    // launch slots [bq_first_slot, total); the persistent kernel takes the rest
+    int bq_first_band = 0;
+    uint64_t bq_first_slot = 0, bq_n_slots = 0;
+    uint32_t bq_n_keys = 0;
+    int bq_key_bits = 1, bq_passes = 32, n_blocks = 0;
+    uint32_t bq_chunk = 1024;
+    uint64_t bq_pass_blocks = 0, bq_finish_blocks = 0;
 };
+
+// Block-queue buffers of a plan, grown to n records of rec_bytes each.
+void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int n_bands)
+{
+    if (fb.bq_cap < n || fb.bq_rec_bytes < rec_bytes) {
+        for (int k = 0; k < 2; ++k) {
+            dfree(fb.bq_rec[k]);
+            dfree(fb.bq_key[k]);
+        }
+        dfree(fb.bq_order);
+        dfree(fb.bq_skey);
+        fb.bq_cap = fb.bq_rec_bytes = 0;
+        for (int k = 0; k < 2; ++k) {
+            fb.bq_rec[k] = dmalloc<unsigned char>(n * rec_bytes);
+            fb.bq_key[k] = dmalloc<uint32_t>(n);
+        }
+        fb.bq_order = dmalloc<int32_t>(n);
+        fb.bq_skey = dmalloc<uint32_t>(n);
+        fb.bq_cap = n;
+        fb.bq_rec_bytes = rec_bytes;
+    }
+    if (fb.bq_seg_cap < (size_t)n_keys + 1) {
+        dfree(fb.bq_seg);
+        fb.bq_seg_cap = 0;
+        fb.bq_seg = dmalloc<uint32_t>((size_t)n_keys + 1);
+        fb.bq_seg_cap = (size_t)n_keys + 1;
+    }
+    if (fb.bq_np_cap < (size_t)n_bands) {
+        dfree(fb.bq_np);
+        fb.bq_np_cap = 0;
+        fb.bq_np = dmalloc<uint32_t>(n_bands);
+        fb.bq_np_cap = n_bands;
+    }
+    const size_t wsb = kp_compact_workspace_bytes((int64_t)n, 1);
+    if (fb.bq_ws_cap < wsb) {
+        dfree(fb.bq_ws);
+        fb.bq_ws_cap = 0;
+        KP_HIP(hipMalloc(&fb.bq_ws, wsb));
+        fb.bq_ws_cap = wsb;
+    }
+    if (!fb.bq_cnt)
+        fb.bq_cnt = dmalloc<unsigned long long>(2);
+}
 
 fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total,
                       const kp_run *run, hipStream_t stream)
@@ -1011,9 +1275,37 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         if (P.img_nrep > 1)
             ensure_replicas(fb.img_rep, fb.img_cap, (uint64_t)P.img_nrep * pl.nI, stream);
     }
+    // block queue: the bands from queue_from_band on
+    pl.n_blocks = e->n_blocks;
+    if (S.kind == KP_SCHED_BLOCKQ) {
+        pl.bq_first_band = std::min(std::max(S.queue_from_band, 0), run->n_bands);
+        for (int b = 0; b < pl.bq_first_band; ++b)
+            pl.bq_first_slot += run->bands[b].n_packets;
+        pl.bq_n_slots = total - pl.bq_first_slot;
+        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 32;
+        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 1024u;
+        pl.bq_pass_blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
+        pl.bq_finish_blocks = (uint64_t)e->n_cu * 4;
+        if (pl.bq_n_slots > 0) {
+            if (e->n3 > kp::bq::kMaxCells)
+                throw kp::error(KP_ERR_UNSUPPORTED, "block queue: blocks of more than 4096 cells");
+            if (pl.bq_n_slots >= (1ull << 31))
+                throw kp::error(KP_ERR_UNSUPPORTED, "block queue: 2^31 packets or more in one launch");
+            const uint64_t nk = (uint64_t)(run->n_bands - pl.bq_first_band) * e->n_blocks;
+            if (nk >= (1ull << 30))
+                throw kp::error(KP_ERR_UNSUPPORTED, "block queue: 2^30 (band, block) keys or more");
+            pl.bq_n_keys = (uint32_t)nk;
+            pl.bq_key_bits = 1;
+            while (pl.bq_key_bits < 30 && (nk - 1) >> pl.bq_key_bits)
+                ++pl.bq_key_bits;
+            const size_t rb = run->precision == KP_F64 ? sizeof(kp::bq::rec_t<double>)
+                                                       : sizeof(kp::bq::rec_t<float>);
+            ensure_bq(fb, pl.bq_n_slots, rb, pl.bq_n_keys, run->n_bands);
+        }
+    }
     // LDS accumulators of the source cell's J and the emission peel-offs
     P.lds_n = 0;
-    if (S.kind == KP_SCHED_PERSISTENT) {
+    if (S.kind == KP_SCHED_PERSISTENT || S.kind == KP_SCHED_BLOCKQ) {
         const uint64_t n = (uint64_t)run->n_bands * (1 + 3 * (uint64_t)run->n_incl);
         if (n * sizeof(double) <= kMaxLdsAcc)
             P.lds_n = (int)n;
@@ -1051,6 +1343,62 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
     P.chunk = (uint32_t)S.chunk;
     P.batch = (uint32_t)S.event_batch;
     return pl;
+}
+
+// The block-queue part of a forward run (kernels only): emission peel-offs
+// and records, queue_passes passes (sort by key, segment table, pass), then
+// the packets still alive to their end.
+template <class Pol>
+void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
+                  hipStream_t stream)
+{
+    kp::bq::qparams Q{};
+    Q.n_keys = pl.bq_n_keys;
+    Q.chunk = pl.bq_chunk;
+    Q.first_band = pl.bq_first_band;
+    Q.n_blocks = pl.n_blocks;
+    Q.first_slot = pl.bq_first_slot;
+    Q.n_slots = pl.bq_n_slots;
+    Q.emit_np = fb.bq_np;
+    const int nqb = P.n_bands - pl.bq_first_band;
+    hipLaunchKernelGGL(k_bq_emit_peel<Pol>, dim3((unsigned)((nqb + 63) / 64)), dim3(64), 0,
+                       stream, P, Q);
+    Q.out = fb.bq_rec[0];
+    Q.out_key = fb.bq_key[0];
+    Q.n_out = fb.bq_cnt + 0;
+    const auto grid_n = [](uint64_t n) {
+        return (unsigned)std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 4096);
+    };
+    hipLaunchKernelGGL(k_bq_generate<Pol>, dim3(grid_n(pl.bq_n_slots)), dim3(256), 0, stream, P,
+                       Q);
+    int cur = 0;
+    for (int pass = 0; pass < pl.bq_passes; ++pass) {
+        const int nxt = 1 - cur;
+        kp::sort_order(reinterpret_cast<const int32_t *>(fb.bq_key[cur]), pl.bq_n_slots,
+                       pl.bq_key_bits, fb.bq_order, fb.bq_ws, fb.bq_ws_cap, stream,
+                       fb.bq_cnt + cur);
+        hipLaunchKernelGGL(k_bq_skey, dim3(grid_n(pl.bq_n_slots)), dim3(256), 0, stream,
+                           fb.bq_key[cur], fb.bq_order, fb.bq_cnt + cur, fb.bq_skey);
+        hipLaunchKernelGGL(k_bq_seg, dim3(grid_n((uint64_t)pl.bq_n_keys + 1)), dim3(256), 0,
+                           stream, fb.bq_skey, fb.bq_cnt + cur, pl.bq_n_keys, fb.bq_seg);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.bq_cnt + nxt);
+        Q.in = fb.bq_rec[cur];
+        Q.in_key = fb.bq_key[cur];
+        Q.order = fb.bq_order;
+        Q.seg = fb.bq_seg;
+        Q.n_in = fb.bq_cnt + cur;
+        Q.out = fb.bq_rec[nxt];
+        Q.out_key = fb.bq_key[nxt];
+        Q.n_out = fb.bq_cnt + nxt;
+        hipLaunchKernelGGL(k_bq_pass<Pol>, dim3((unsigned)pl.bq_pass_blocks), dim3(256),
+                           (size_t)P.n3 * (sizeof(double) + sizeof(float)), stream, P, Q);
+        cur = nxt;
+    }
+    Q.in = fb.bq_rec[cur];
+    Q.in_key = fb.bq_key[cur];
+    Q.n_in = fb.bq_cnt + cur;
+    hipLaunchKernelGGL(k_bq_finish<Pol>, dim3((unsigned)pl.bq_finish_blocks), dim3(256), 0,
+                       stream, P, Q);
 }
 
 // Launches of a planned forward run on `stream`: n_chunks transport kernels
@@ -1091,13 +1439,26 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
                 hipLaunchKernelGGL(k_forward_f32, dim3((unsigned)pl.blocks), dim3(pl.T), 0,
                                    stream, P);
         } else {
-            // mixed precision in emission-direction order: the lockstep variant
-            const persist_fn f = (run->precision == KP_F32 && P.order)
-                                     ? sync_variant()
-                                     : persist_variant<false>(run->precision, S.waves_per_simd);
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.work);
-            hipLaunchKernelGGL(f, dim3((unsigned)pl.blocks), dim3(pl.T),
-                               (size_t)P.lds_n * sizeof(double), stream, P);
+            // the persistent kernel: every band, or those before the block queue
+            kp::kparams Pp = P;
+            if (S.kind == KP_SCHED_BLOCKQ)
+                Pp.total = pl.bq_first_slot;
+            if (Pp.total > 0) {
+                // mixed precision in emission-direction order: the lockstep variant
+                const persist_fn f =
+                    (run->precision == KP_F32 && P.order)
+                        ? sync_variant()
+                        : persist_variant<false>(run->precision, S.waves_per_simd);
+                hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.work);
+                hipLaunchKernelGGL(f, dim3((unsigned)pl.blocks), dim3(pl.T),
+                                   (size_t)P.lds_n * sizeof(double), stream, Pp);
+            }
+            if (S.kind == KP_SCHED_BLOCKQ && pl.bq_n_slots > 0) {
+                if (run->precision == KP_F64)
+                    issue_blockq<kp::pol64>(fb, P, pl, stream);
+                else
+                    issue_blockq<kp::pol32>(fb, P, pl, stream);
+            }
         }
     }
     if (J_out && pl.j_band_major)
@@ -1804,6 +2165,40 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 try_knob(&kp_schedule::waves_per_simd, {0, 1, 5});
             try_knob(&kp_schedule::blocks_per_cu, {2, 4, 8});
             try_knob(&kp_schedule::block_threads, {128, 256});
+            // block queue for the optically thinnest bands: the longest tail
+            // of bands whose extinction scale is at most t, for a few t
+            // (every band at t = infinity); the bands before it stay on the
+            // persistent kernel
+            if (!sph_mesh && e->n3 <= kp::bq::kMaxCells && total < (1ull << 31)) {
+                const kp_schedule base = cur;
+                float queue_best = best_ms;
+                kp_schedule queue_pick = cur;
+                std::vector<int> firsts;
+                for (double t : {1e300, 1.0, 0.5, 0.25, 0.1}) {
+                    int k = run->n_bands;
+                    while (k > 0 && run->bands[k - 1].kappa_scale <= t)
+                        --k;
+                    if (k < run->n_bands &&
+                        std::find(firsts.begin(), firsts.end(), k) == firsts.end())
+                        firsts.push_back(k);
+                }
+                for (int k : firsts) {
+                    kp_schedule c = base;
+                    c.kind = KP_SCHED_BLOCKQ;
+                    c.queue_from_band = k;
+                    const float ms = time_it(c);
+                    if (ms < queue_best) {
+                        queue_best = ms;
+                        queue_pick = c;
+                    }
+                }
+                if (queue_best < best_ms) {
+                    best_ms = queue_best;
+                    cur = queue_pick;
+                    try_knob(&kp_schedule::queue_chunk, {256, 4096});
+                    try_knob(&kp_schedule::queue_passes, {16, 64});
+                }
+            }
             e->sched = cur;
             e->tuned[key.str()] = cur;
             *best = cur;
@@ -1821,8 +2216,19 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
     return kp::guard([&] {
         if (!e || !s)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: null argument");
-        if (s->kind != KP_SCHED_HISTORY && s->kind != KP_SCHED_PERSISTENT)
+        if (s->kind != KP_SCHED_HISTORY && s->kind != KP_SCHED_PERSISTENT &&
+            s->kind != KP_SCHED_BLOCKQ)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: unknown kind");
+        if (s->kind == KP_SCHED_BLOCKQ && e->coord == KP_COORD_SPHERICAL)
+            throw kp::error(KP_ERR_UNSUPPORTED,
+                            "kp_set_schedule: no block queue on spherical meshes");
+        if (s->kind == KP_SCHED_BLOCKQ && e->n3 > kp::bq::kMaxCells)
+            throw kp::error(KP_ERR_UNSUPPORTED,
+                            "kp_set_schedule: block queue needs n_cell^3 <= 4096");
+        if (s->queue_from_band < 0 || s->queue_passes < 0 || s->queue_passes > 4096 ||
+            s->queue_chunk < 0 || s->queue_chunk > (1 << 20))
+            throw kp::error(KP_ERR_ARG, "kp_set_schedule: queue_from_band >= 0, queue_passes in "
+                                        "[0,4096], queue_chunk in [0,2^20]");
         if (s->block_threads != 64 && s->block_threads != 128 && s->block_threads != 256)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: block_threads must be 64, 128 or 256");
         if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64)

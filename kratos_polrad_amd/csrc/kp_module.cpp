@@ -303,7 +303,15 @@ struct kp_module {
         const std::string pr = ini.get<std::string>("photon", "precision", "f64");
         precision = pr == "f32" ? KP_F32 : KP_F64;
         const std::string sk = ini.get<std::string>("photon", "schedule", "persistent");
-        sched_kind = sk == "history" ? KP_SCHED_HISTORY : KP_SCHED_PERSISTENT;
+        if (sk == "history")
+            sched_kind = KP_SCHED_HISTORY;
+        else if (sk == "persistent")
+            sched_kind = KP_SCHED_PERSISTENT;
+        else if (sk == "block_queue")
+            sched_kind = KP_SCHED_BLOCKQ;
+        else
+            throw kp::error(KP_ERR_ARG,
+                            "ini: photon|schedule must be persistent, history or block_queue");
         sched_given = ini.has("photon", "schedule");
         if (ini.has("photon", "order")) {
             const std::string o = ini.get<std::string>("photon", "order", "slot");

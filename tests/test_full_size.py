@@ -99,6 +99,51 @@ def test_config2_mixed_tracks_fp64_on_the_same_packets():
         assert _rel(r32["image"][..., k], r64["image"][..., k]) < 2e-3
 
 
+def _batch_J(eng, w, run, n_batches, per, offset):
+    """Per-cell mean and standard error of J over independent batches, and a
+    mask of the cells that every batch reached."""
+    import torch
+
+    s = s2 = None
+    hit = None
+    for b in range(n_batches):
+        r = RunSpec(**{**run.__dict__, "packet_offset": offset + b * per})
+        out = eng.alloc_outputs(r, image=False, counters=False)
+        eng.forward(w.source, r, out)
+        J = out["J"]
+        s = J.clone() if s is None else s + J
+        s2 = J * J if s2 is None else s2 + J * J
+        hit = (J > 0) if hit is None else (hit & (J > 0))
+    torch.cuda.synchronize()
+    mean = s / n_batches
+    var = (s2 / n_batches - mean * mean).clamp(min=0) * (n_batches / (n_batches - 1))
+    return mean.cpu().numpy(), (var / n_batches).sqrt().cpu().numpy(), hit.cpu().numpy()
+
+
+def test_config2_mixed_J_per_cell_within_3_sigma_of_independent_fp64():
+    """Per tally: the mixed path's J agrees cell by cell with an FP64 run on
+    other packet ids within 3 combined standard errors (8 batches each), as
+    often as two FP64 runs on different ids do."""
+    w = configs.config2(n_packets=1, precision=_abi.KP_F64)
+    eng = Engine(w.mesh)
+    eng.set_media(w.kappa, w.albedo, w.g)
+    per = 250_000
+    run64 = RunSpec(**{**w.run.__dict__, "bands": [type(w.run.bands[0])(
+        **{**w.run.bands[0].__dict__, "n_packets": per})]})
+    run32 = RunSpec(**{**run64.__dict__, "precision": _abi.KP_F32})
+    m32, e32, h32 = _batch_J(eng, w, run32, 8, per, 0)
+    m64, e64, h64 = _batch_J(eng, w, run64, 8, per, 1 << 40)
+    m64b, e64b, h64b = _batch_J(eng, w, run64, 8, per, 1 << 41)
+    sel = h32 & h64 & h64b
+    assert sel.sum() > 100_000  # cells every batch of every run reached
+    within = np.abs(m32 - m64)[sel] <= 3 * np.sqrt(e32 ** 2 + e64 ** 2)[sel]
+    control = np.abs(m64b - m64)[sel] <= 3 * np.sqrt(e64b ** 2 + e64 ** 2)[sel]
+    print(f"cells {sel.sum()}: mixed within 3 sigma {within.mean():.4f}, "
+          f"FP64 control {control.mean():.4f}", flush=True)
+    assert within.mean() >= 0.97
+    assert within.mean() >= control.mean() - 0.01
+
+
 def test_config5_full_mesh_accounting_and_tracking():
     w = configs.config5(n_packets=1_000_000, precision=_abi.KP_F32)
     assert w.mesh.n_blocks > 25_000
