@@ -164,9 +164,11 @@ def adjoint_ratio(n):
 
 
 def batched(steps, scale):
-    """64-band batched forward (config 3): autotuned schedule, all bands in one
-    persistent launch captured into a hipGraph; mixed precision vs the FP64
-    path on the same packets, each in its faster packet order at full size."""
+    """64-band batched forward (config 3): all bands in one launch captured
+    into a hipGraph, each precision on its own fastest schedule (the
+    autotuner's pick, screened at full size against its other packet order
+    and against the persistent kernel or the block queue for the optically
+    thin bands); mixed precision vs the FP64 path on the same packets."""
     import numpy as np
     import torch
 
@@ -181,17 +183,13 @@ def batched(steps, scale):
            "max_band_packets": max(b.n_packets for b in w.run.bands)}
     res["autotune_s"] = 0.0
 
-    def run_graph(prec, offset=0, order=None):
+    first_thin = next((k for k, b in enumerate(w.run.bands) if b.kappa_scale <= 0.5),
+                      len(w.run.bands))
+    res["first_band_kappa_scale_le_0.5"] = first_thin
+
+    def run_graph(prec, sched, reps, offset=0):
         run = RunSpec(**{**w.run.__dict__, "precision": prec, "packet_offset": offset})
-        # each precision runs with its own tuned schedule (cached per precision),
-        # in the packet order given (None: as tuned)
-        t0 = time.perf_counter()
-        sched = eng.autotune(w.source, run, sample_packets=10_000_000)
-        res["autotune_s"] += time.perf_counter() - t0
-        progress(f"batched: precision {prec} tuned in {time.perf_counter() - t0:.1f} s")
-        if order is not None:
-            sched = {**sched, "coherent": order}
-            eng.set_schedule(**sched)
+        eng.set_schedule(**sched)
         out = eng.alloc_outputs(run)
         g = eng.graph(w.source, run, out, n_chunks=1)
         g.launch()  # warm-up replay
@@ -200,37 +198,58 @@ def batched(steps, scale):
             v.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(reps):
             g.launch()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        progress(f"batched: precision {prec}, order {order}, {1e3 * wall / steps:.0f} ms per step")
         cnt = out["counters"].cpu().numpy().astype(np.float64)
         img = out["image"].cpu().numpy()[:, 0]
         del g
-        return wall, cnt, img, sched
+        return wall, cnt, img
 
-    def best_order(prec):
-        # the tuner decides the packet order on a sample; the order's effect
-        # on J-address collisions grows with the packet density, so both
-        # orders are timed again at full size and the faster one counts
-        runs = [run_graph(prec, order=o) for o in (0, 1)]
+    def best_schedule(prec):
+        # the tuner decides on a 1e7-packet sample; packet order and the block
+        # queue's share pay off differently at full size (J-address collisions
+        # and packets per block grow with the packet density), so the tuned
+        # schedule and its alternatives are screened again at full size with
+        # two replays each, and the fastest is timed
+        run = RunSpec(**{**w.run.__dict__, "precision": prec})
+        t0 = time.perf_counter()
+        tuned = eng.autotune(w.source, run, sample_packets=10_000_000)
+        res["autotune_s"] += time.perf_counter() - t0
+        progress(f"batched: precision {prec} tuned in {time.perf_counter() - t0:.1f} s "
+                 f"(kind {tuned['kind']}, queue from band {tuned['queue_from_band']})")
+        cands = {"tuned": tuned, "tuned_other_order": {**tuned, "coherent": 1 - tuned["coherent"]}}
+        if tuned["kind"] == _abi.KP_SCHED_BLOCKQ:
+            cands["persistent"] = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
+        else:
+            for k in sorted({0, first_thin}):
+                if k < len(w.run.bands):
+                    cands[f"block_queue_from_{k}"] = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ,
+                                                      "queue_from_band": k}
+        screen = {}
+        for name, sc in cands.items():
+            screen[name] = 1e3 * run_graph(prec, sc, 2)[0] / 2
+            progress(f"batched: precision {prec}, {name}: {screen[name]:.0f} ms per step")
+        pick = min(screen, key=screen.get)
         key = "mixed" if prec == _abi.KP_F32 else "fp64"
-        res["order_check_ms_" + key] = [1e3 * r[0] / steps for r in runs]
-        best = min(runs, key=lambda r: r[0])
-        res["schedule_" + key] = best[3]
-        return best[:3]
+        res["screen_ms_" + key] = screen
+        res["schedule_" + key] = cands[pick]
+        wall, cnt, img = run_graph(prec, cands[pick], steps)
+        progress(f"batched: precision {prec}, {pick}: {1e3 * wall / steps:.0f} ms per step "
+                 f"over {steps} steps")
+        return wall, cnt, img
 
-    wall32, cnt32, img32 = best_order(_abi.KP_F32)
+    wall32, cnt32, img32 = best_schedule(_abi.KP_F32)
     res["value"] = float((cnt32[0] + cnt32[1]) / wall32)
     res["ms_per_step"] = 1e3 * wall32 / steps
-    wall64, cnt64, img64 = best_order(_abi.KP_F64)
+    wall64, cnt64, img64 = best_schedule(_abi.KP_F64)
     res["fp64_value"] = float((cnt64[0] + cnt64[1]) / wall64)
     res["speedup_mixed_vs_fp64"] = res["value"] / res["fp64_value"]
     # image error per band against an independent FP64 run (other packet
     # ids), relative to the band's I norm: mixed and FP64 must show the same
     # error (the Monte Carlo noise); median and worst band reported
-    _, _, ref, _ = run_graph(_abi.KP_F64, offset=1 << 40)
+    _, _, ref = run_graph(_abi.KP_F64, res["schedule_fp64"], steps, offset=1 << 40)
     e32 = np.array([[img_err(img32[b], ref[b])[k] for k in "IQU"] for b in range(len(ref))])
     e64 = np.array([[img_err(img64[b], ref[b])[k] for k in "IQU"] for b in range(len(ref))])
     for name, e in (("mixed_vs_independent_fp64", e32), ("fp64_vs_independent_fp64", e64)):

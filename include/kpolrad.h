@@ -247,8 +247,10 @@ typedef struct kp_tallies {
  *   (interactions and peel-offs included), then adds the tile to J with one
  *   atomic per touched cell; packets that cross into another block are
  *   compacted into the next pass's records by wave ballot and prefix rank.
- *   After queue_passes passes the packets still alive finish one thread
- *   each.  Bands before queue_from_band run on the persistent kernel in the
+ *   A packet that takes queue_yield steps in one pass without leaving its
+ *   block goes back to the queue.  Peel-off rays are queued and marched by
+ *   a kernel of their own after each pass, one ray per lane.  After
+ *   queue_passes passes the packets still alive finish one thread each.  Bands before queue_from_band run on the persistent kernel in the
  *   same launch.  Same histories; needs n_cell^3 <= 4096 and fewer than 2^31
  *   queued packets per launch.  Record memory: 2 x record size + 16 B per
  *   queued packet, kept by the engine. */
@@ -317,11 +319,14 @@ typedef struct kp_schedule {
     int32_t adjoint_chunk;
     /* KP_SCHED_BLOCKQ: first band of the block queue (0 = every band; the
      * bands before it run on the persistent kernel), passes before the
-     * packets still alive finish one thread each (0 = 32), sorted packets
-     * per workgroup task (0 = 1024). */
+     * packets still alive finish one thread each (0 = 16), sorted packets
+     * per workgroup task (0 = 256), and the march steps plus interactions a
+     * packet may take in one pass before it goes back to the queue in its
+     * block (0 = 128), which bounds the slowest lane of a pass. */
     int32_t queue_from_band;
     int32_t queue_passes;
     int32_t queue_chunk;
+    int32_t queue_yield;
 } kp_schedule;
 
 kp_status kp_set_schedule(kp_engine *eng, const kp_schedule *sched);

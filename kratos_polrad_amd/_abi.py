@@ -112,6 +112,7 @@ class Schedule(C.Structure):
         ("queue_from_band", C.c_int32),
         ("queue_passes", C.c_int32),
         ("queue_chunk", C.c_int32),
+        ("queue_yield", C.c_int32),
     ]
 
 

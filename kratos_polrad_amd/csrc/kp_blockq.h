@@ -55,6 +55,18 @@ template <> struct __align__(16) rec_t<float> {
 };
 static_assert(sizeof(rec_t<double>) == 144 && sizeof(rec_t<float>) == 96, "record sizes");
 
+// A peel-off ray of a queued packet, marched by the peel kernel after the
+// pass: its start (the interaction point: position as in rec_t, block,
+// cell), observer, packet slot, and the Stokes vector it carries before
+// attenuation.
+template <class R> struct __align__(16) ray_rec_t {
+    R x[3];
+    int32_t ib;
+    uint32_t cell_o;                 // i0 | i1 << 8 | i2 << 16 | observer << 24
+    uint32_t slot;
+    R s[3];
+};
+
 // Launch parameters of the queue kernels (besides kparams).
 struct qparams {
     const void *in;                   // records of this pass
@@ -67,11 +79,16 @@ struct qparams {
     unsigned long long *n_out;        // claimed out records (zeroed before the pass)
     uint32_t n_keys;                  // queue bands x blocks
     uint32_t chunk;                   // sorted positions per workgroup task
+    uint32_t yield;                   // steps + interactions of a packet per pass
     int first_band;                   // key = (band - first_band) * n_blocks + block
     int n_blocks;
     uint64_t first_slot;              // first launch slot of the queue bands
     uint64_t n_slots;                 // slots of the queue bands
     uint32_t *emit_np;                // [n_bands] peel-off crossings of the emission peel-offs
+    // peel-off rays of this pass (null: every lane marches its own)
+    void *pr;
+    unsigned long long *pr_count;
+    uint64_t pr_cap;
 };
 
 // A packet between march steps.
@@ -149,7 +166,50 @@ __device__ __forceinline__ void store(const qlane<Pol> &L, rec_t<typename Pol::r
 // the domain edge through HBM kappa_rho), the scattering and Russian
 // roulette.  The operations and their order are those of the other
 // schedules.  Returns false when roulette ends the packet.
-template <class Pol> __device__ bool interact(const kparams &P, qlane<Pol> &L, int64_t gc)
+// One queue slot for each calling lane (called by those lanes together): one
+// atomic per wave, lanes ranked by ballot.
+__device__ __forceinline__ uint64_t claim(unsigned long long *count)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t rank = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+// Queue the peel-off ray toward observer o (Stokes vector S before
+// attenuation) for the peel kernel; false when the queue is full.  Called
+// by the interacting lanes together.
+template <class Pol>
+__device__ __forceinline__ bool push_ray(const qparams &Q, const qlane<Pol> &L, int o,
+                                         typename Pol::real S0, typename Pol::real S1,
+                                         typename Pol::real S2)
+{
+    using real = typename Pol::real;
+    const uint64_t k = claim(Q.pr_count);
+    if (k >= Q.pr_cap)
+        return false;
+    ray_rec_t<real> r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        r.x[a] = L.r.x[a];
+    r.ib = L.r.ib;
+    r.cell_o = (uint32_t)L.r.i[0] | ((uint32_t)L.r.i[1] << 8) | ((uint32_t)L.r.i[2] << 16) |
+               ((uint32_t)o << 24);
+    r.slot = (uint32_t)L.slot;
+    r.s[0] = S0;
+    r.s[1] = S1;
+    r.s[2] = S2;
+    static_cast<ray_rec_t<real> *>(Q.pr)[k] = r;
+    return true;
+}
+
+template <class Pol>
+__device__ bool interact(const kparams &P, const qparams &Qp, qlane<Pol> &L, int64_t gc)
 {
     using real = typename Pol::real;
     L.events += 1;
@@ -169,6 +229,8 @@ template <class Pol> __device__ bool interact(const kparams &P, qlane<Pol> &L, i
             double Ip, Qi, Ui;
             f64::peel_stokes(P.model, gg, P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, P.no[o],
                              P.ev[o], Ip, Qi, Ui);
+            if (Qp.pr && push_ray<Pol>(Qp, L, o, Ip, Qi, Ui))
+                continue;
             const double tau = f64::tau_to_edge(P, L.r, P.no[o], L.ks, np);
             const double att = kp_exp(-tau) * KP_INV_4PI;
             f64::deposit(P, L.band, o, L.r.x, Ip * att, Qi * att, Ui * att);
@@ -205,6 +267,8 @@ template <class Pol> __device__ bool interact(const kparams &P, qlane<Pol> &L, i
                 float Ip, Qi, Ui;
                 f32::peel_stokes(P.model, gg, (float)P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, no,
                                  ev, Ip, Qi, Ui);
+                if (Qp.pr && push_ray<Pol>(Qp, L, o, Ip, Qi, Ui))
+                    continue;
                 const double tau = f32::tau_to_edge(P, L.r, no, L.ks, np);
                 const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
                 f32::deposit(P, L.band, o, gx, Ip * att, Qi * att, Ui * att);
@@ -236,15 +300,25 @@ enum : int { MARCH_END = 0, MARCH_OUT = 1 };
 // The packet's march from its current point.  LOCAL: kappa_rho from the LDS
 // tile lk of the packet's block and J into the LDS tile lj; returns
 // MARCH_OUT as soon as the ray has crossed into another block (the lane then
-// holds it in the new block).  Otherwise kappa_rho from HBM and J by global
+// holds it in the new block), or after Q.yield steps and interactions in
+// this block (it goes back to the queue where it stands).  Otherwise kappa_rho from HBM and J by global
 // atomics, to the packet's end.  MARCH_END: L.status holds the end status.
 template <class Pol, bool LOCAL>
-__device__ int march(const kparams &P, qlane<Pol> &L, const float *lk, double *lj)
+__device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
+                     double *lj)
 {
     using real = typename Pol::real;
+    uint32_t work = 0; // steps and interactions in this pass (LOCAL)
     for (;;) {
         int64_t gc = 0;
         for (;;) {
+            if constexpr (LOCAL) {
+                // back to the queue, still in this block: every packet's
+                // share of a pass stays bounded
+                if (work >= Q.yield)
+                    return MARCH_OUT;
+                ++work;
+            }
             if (L.steps <= 0) {
                 L.status = KP_END_STEPS;
                 return MARCH_END;
@@ -306,10 +380,12 @@ __device__ int march(const kparams &P, qlane<Pol> &L, const float *lk, double *l
             if constexpr (LOCAL)
                 return MARCH_OUT;
         }
-        if (!interact<Pol>(P, L, gc)) {
+        if (!interact<Pol>(P, Q, L, gc)) {
             L.status = KP_END_RR;
             return MARCH_END;
         }
+        if constexpr (LOCAL)
+            ++work;
     }
 }
 
@@ -346,9 +422,49 @@ __device__ __forceinline__ void end_packet(const kparams &P, const qlane<Pol> &L
         es[KP_ES_STATUS] = (double)L.status;
         es[KP_ES_IB] = (double)L.r.ib;
         es[KP_ES_CELL] = (double)Pol::cell(P, L.r);
-        es[14] = (double)L.npeel;
+        // the peel kernel adds the crossings of the rays it marched for this
+        // packet, before or after this (the generator cleared the entry)
+        atomicAdd(&es[14], (double)L.npeel);
         es[15] = 0.0;
     }
+}
+
+// A queued peel-off ray marched to the domain edge (kappa_rho from HBM), its
+// attenuated Stokes vector deposited at the pixel of its start.  Returns the
+// ray's crossings.
+template <class Pol>
+__device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<typename Pol::real> &q)
+{
+    using real = typename Pol::real;
+    typename Pol::ray t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        t.x[a] = q.x[a];
+        t.d[a] = 0;
+    }
+    t.ib = q.ib;
+    unpack_cell<Pol>(t, q.cell_o);
+    const int o = (int)(q.cell_o >> 24);
+    const int band = band_of(P, q.slot);
+    const real ks = (real)P.bands[band].kappa_scale;
+    uint64_t np = 0;
+    if constexpr (sizeof(real) == 8) {
+        f64::gl_load(P, t);
+        const double tau = f64::tau_to_edge(P, t, P.no[o], ks, np);
+        const double att = kp_exp(-tau) * KP_INV_4PI;
+        f64::deposit(P, band, o, t.x, q.s[0] * att, q.s[1] * att, q.s[2] * att);
+    } else {
+        float no[3];
+        load3(P.no[o], no);
+        float gx[3];
+        f32::global_pos(P, t, gx);
+        const double tau = f32::tau_to_edge(P, t, no, ks, np);
+        const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
+        f32::deposit(P, band, o, gx, q.s[0] * att, q.s[1] * att, q.s[2] * att);
+    }
+    if (P.end_state)
+        atomicAdd(&P.end_state[(uint64_t)q.slot * KP_ES_STRIDE + 14], (double)np);
+    return (uint32_t)np;
 }
 
 // Record of the next pass for each calling lane that crossed into another
@@ -359,15 +475,7 @@ template <class Pol>
 __device__ __forceinline__ void push_out(const kparams &P, const qparams &Q, const qlane<Pol> &L)
 {
     using real = typename Pol::real;
-    const int lane = threadIdx.x & 63;
-    const uint64_t m = __ballot(1);
-    const int leader = __ffsll((long long)m) - 1;
-    const uint32_t rank = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-    unsigned long long base = 0;
-    if (lane == leader)
-        base = atomicAdd(Q.n_out, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    const uint64_t k = base + rank;
+    const uint64_t k = claim(Q.n_out);
     store<Pol>(L, static_cast<rec_t<real> *>(Q.out)[k]);
     Q.out_key[k] = (uint32_t)(L.band - Q.first_band) * (uint32_t)Q.n_blocks + (uint32_t)L.r.ib;
 }

@@ -53,8 +53,10 @@ struct fwd_bufs {
     int32_t *bq_order = nullptr;
     uint32_t *bq_skey = nullptr, *bq_seg = nullptr, *bq_np = nullptr;
     void *bq_ws = nullptr;
-    unsigned long long *bq_cnt = nullptr;
+    unsigned long long *bq_cnt = nullptr; // [2] records, [2] queued peel-off rays
     size_t bq_cap = 0, bq_rec_bytes = 0, bq_seg_cap = 0, bq_np_cap = 0, bq_ws_cap = 0;
+    void *bq_rays = nullptr;
+    size_t bq_rays_bytes = 0;
 };
 
 struct kp_engine {
@@ -476,6 +478,8 @@ __global__ void __launch_bounds__(256) k_bq_generate(kp::kparams P, kp::bq::qpar
         L.events = 0;
         L.ncross = 0;
         L.npeel = Q.emit_np[L.band];
+        if (P.end_state) // peel crossings are added to it as the rays are marched
+            P.end_state[L.slot * KP_ES_STRIDE + 14] = 0.0;
         kp::bq::store<Pol>(L, static_cast<kp::bq::rec_t<real> *>(Q.out)[k]);
         Q.out_key[k] = (uint32_t)(L.band - Q.first_band) * (uint32_t)Q.n_blocks + (uint32_t)P.src_ib;
     }
@@ -555,7 +559,7 @@ __global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams 
             for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
                 kp::bq::qlane<Pol> L;
                 kp::bq::load<Pol>(P, in[Q.order[q]], L);
-                if (kp::bq::march<Pol, true>(P, L, lk, lj) == kp::bq::MARCH_OUT)
+                if (kp::bq::march<Pol, true>(P, Q, L, lk, lj) == kp::bq::MARCH_OUT)
                     kp::bq::push_out<Pol>(P, Q, L);
                 else
                     kp::bq::end_packet<Pol>(P, L, C);
@@ -576,6 +580,21 @@ __global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams 
     flush_counters(P, C);
 }
 
+// The peel-off rays queued by a pass: one ray per lane, in queue order (the
+// rays a wave queued start in one block and run parallel to one observer).
+template <class Pol>
+__global__ void __launch_bounds__(256) k_bq_peel(kp::kparams P, kp::bq::qparams Q)
+{
+    using real = typename Pol::real;
+    kp::lane_counters C;
+    const uint64_t n = *Q.pr_count < Q.pr_cap ? *Q.pr_count : Q.pr_cap;
+    const auto *rays = static_cast<const kp::bq::ray_rec_t<real> *>(Q.pr);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
+    flush_counters(P, C);
+}
+
 // Packets still alive after the last pass: each runs to its end in one thread.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparams Q)
@@ -588,7 +607,7 @@ __global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparam
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         kp::bq::qlane<Pol> L;
         kp::bq::load<Pol>(P, in[k], L);
-        (void)kp::bq::march<Pol, false>(P, L, nullptr, nullptr);
+        (void)kp::bq::march<Pol, false>(P, Q, L, nullptr, nullptr);
         kp::bq::end_packet<Pol>(P, L, C);
     }
     flush_counters(P, C);
@@ -899,7 +918,9 @@ void free_bufs(fwd_bufs &b)
     dfree(b.bq_np);
     dfree(b.bq_ws);
     dfree(b.bq_cnt);
+    dfree(b.bq_rays);
     b.bq_cap = b.bq_rec_bytes = b.bq_seg_cap = b.bq_np_cap = b.bq_ws_cap = 0;
+    b.bq_rays_bytes = 0;
 }
 
 void free_all(kp_engine *e)
@@ -1182,8 +1203,9 @@ struct fwd_plan {
     uint64_t bq_first_slot = 0, bq_n_slots = 0;
     uint32_t bq_n_keys = 0;
     int bq_key_bits = 1, bq_passes = 32, n_blocks = 0;
-    uint32_t bq_chunk = 1024;
+    uint32_t bq_chunk = 256, bq_yield = 128;
     uint64_t bq_pass_blocks = 0, bq_finish_blocks = 0;
+    uint64_t bq_ray_cap = 0; // peel-off rays a pass can queue (0: marched in place)
 };
 
 // Block-queue buffers of a plan, grown to n records of rec_bytes each.
@@ -1226,7 +1248,18 @@ void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int 
         fb.bq_ws_cap = wsb;
     }
     if (!fb.bq_cnt)
-        fb.bq_cnt = dmalloc<unsigned long long>(2);
+        fb.bq_cnt = dmalloc<unsigned long long>(3);
+}
+
+// Peel-off ray queue of the block queue: `rays` records of ray_bytes each.
+void ensure_bq_rays(fwd_bufs &fb, uint64_t rays, size_t ray_bytes)
+{
+    if (fb.bq_rays_bytes >= rays * ray_bytes)
+        return;
+    dfree(fb.bq_rays);
+    fb.bq_rays_bytes = 0;
+    fb.bq_rays = dmalloc<unsigned char>(rays * ray_bytes);
+    fb.bq_rays_bytes = rays * ray_bytes;
 }
 
 fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total,
@@ -1282,8 +1315,9 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         for (int b = 0; b < pl.bq_first_band; ++b)
             pl.bq_first_slot += run->bands[b].n_packets;
         pl.bq_n_slots = total - pl.bq_first_slot;
-        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 32;
-        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 1024u;
+        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 16;
+        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 256u;
+        pl.bq_yield = S.queue_yield > 0 ? (uint32_t)S.queue_yield : 128u;
         pl.bq_pass_blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
         pl.bq_finish_blocks = (uint64_t)e->n_cu * 4;
         if (pl.bq_n_slots > 0) {
@@ -1301,6 +1335,18 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
             const size_t rb = run->precision == KP_F64 ? sizeof(kp::bq::rec_t<double>)
                                                        : sizeof(kp::bq::rec_t<float>);
             ensure_bq(fb, pl.bq_n_slots, rb, pl.bq_n_keys, run->n_bands);
+            // peel-off rays of a pass: twice one ray per packet and observer,
+            // at least 2^20 and at most 2^27 rays or 8 GB; rays beyond it are
+            // marched by the lane that made them
+            if (run->n_incl > 0) {
+                const size_t rr = run->precision == KP_F64 ? sizeof(kp::bq::ray_rec_t<double>)
+                                                           : sizeof(kp::bq::ray_rec_t<float>);
+                uint64_t cap = 2 * pl.bq_n_slots * (uint64_t)run->n_incl;
+                cap = std::max<uint64_t>(cap, 1ull << 20);
+                cap = std::min<uint64_t>(cap, std::min<uint64_t>(1ull << 27, (8ull << 30) / rr));
+                pl.bq_ray_cap = cap;
+                ensure_bq_rays(fb, cap, rr);
+            }
         }
     }
     // LDS accumulators of the source cell's J and the emission peel-offs
@@ -1355,11 +1401,18 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     kp::bq::qparams Q{};
     Q.n_keys = pl.bq_n_keys;
     Q.chunk = pl.bq_chunk;
+    Q.yield = pl.bq_yield;
     Q.first_band = pl.bq_first_band;
     Q.n_blocks = pl.n_blocks;
     Q.first_slot = pl.bq_first_slot;
     Q.n_slots = pl.bq_n_slots;
     Q.emit_np = fb.bq_np;
+    Q.pr = pl.bq_ray_cap > 0 ? fb.bq_rays : nullptr;
+    Q.pr_count = fb.bq_cnt + 2;
+    Q.pr_cap = pl.bq_ray_cap;
+    const unsigned peel_blocks = (unsigned)pl.bq_finish_blocks * 2;
+    if (Q.pr)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);
     const int nqb = P.n_bands - pl.bq_first_band;
     hipLaunchKernelGGL(k_bq_emit_peel<Pol>, dim3((unsigned)((nqb + 63) / 64)), dim3(64), 0,
                        stream, P, Q);
@@ -1392,6 +1445,10 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
         Q.n_out = fb.bq_cnt + nxt;
         hipLaunchKernelGGL(k_bq_pass<Pol>, dim3((unsigned)pl.bq_pass_blocks), dim3(256),
                            (size_t)P.n3 * (sizeof(double) + sizeof(float)), stream, P, Q);
+        if (Q.pr) {
+            hipLaunchKernelGGL(k_bq_peel<Pol>, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);
+        }
         cur = nxt;
     }
     Q.in = fb.bq_rec[cur];
@@ -1399,6 +1456,8 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     Q.n_in = fb.bq_cnt + cur;
     hipLaunchKernelGGL(k_bq_finish<Pol>, dim3((unsigned)pl.bq_finish_blocks), dim3(256), 0,
                        stream, P, Q);
+    if (Q.pr)
+        hipLaunchKernelGGL(k_bq_peel<Pol>, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
 }
 
 // Launches of a planned forward run on `stream`: n_chunks transport kernels
@@ -2195,8 +2254,10 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 if (queue_best < best_ms) {
                     best_ms = queue_best;
                     cur = queue_pick;
-                    try_knob(&kp_schedule::queue_chunk, {256, 4096});
-                    try_knob(&kp_schedule::queue_passes, {16, 64});
+                    try_knob(&kp_schedule::queue_chunk, {128, 512});
+                    try_knob(&kp_schedule::queue_passes, {8, 24, 32});
+                    try_knob(&kp_schedule::queue_yield, {64, 256});
+                    try_knob(&kp_schedule::blocks_per_cu, {4, 8, 16});
                 }
             }
             e->sched = cur;
@@ -2226,9 +2287,9 @@ kp_status kp_set_schedule(kp_engine *e, const kp_schedule *s)
             throw kp::error(KP_ERR_UNSUPPORTED,
                             "kp_set_schedule: block queue needs n_cell^3 <= 4096");
         if (s->queue_from_band < 0 || s->queue_passes < 0 || s->queue_passes > 4096 ||
-            s->queue_chunk < 0 || s->queue_chunk > (1 << 20))
+            s->queue_chunk < 0 || s->queue_chunk > (1 << 20) || s->queue_yield < 0)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: queue_from_band >= 0, queue_passes in "
-                                        "[0,4096], queue_chunk in [0,2^20]");
+                                        "[0,4096], queue_chunk in [0,2^20], queue_yield >= 0");
         if (s->block_threads != 64 && s->block_threads != 128 && s->block_threads != 256)
             throw kp::error(KP_ERR_ARG, "kp_set_schedule: block_threads must be 64, 128 or 256");
         if (s->blocks_per_cu < 1 || s->blocks_per_cu > 64)

@@ -110,12 +110,12 @@ class Engine:
                      chunk=64, j_replicas=0, img_replicas=0, event_batch=8,
                      waves_per_simd=0, coherent=0, peel_queue=1, vertex_slots=0,
                      adjoint_chunk=0, queue_from_band=0, queue_passes=0,
-                     queue_chunk=0) -> None:
+                     queue_chunk=0, queue_yield=0) -> None:
         s = _abi.Schedule(int(kind), int(block_threads), int(blocks_per_cu), int(chunk),
                           int(j_replicas), int(img_replicas), int(event_batch),
                           int(waves_per_simd), int(coherent), int(peel_queue),
                           int(vertex_slots), int(adjoint_chunk), int(queue_from_band),
-                          int(queue_passes), int(queue_chunk))
+                          int(queue_passes), int(queue_chunk), int(queue_yield))
         _abi.check(_abi.lib().kp_set_schedule(self._h, C.byref(s)))
 
     def get_schedule(self) -> dict:

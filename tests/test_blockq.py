@@ -8,8 +8,9 @@ the same histories as the other schedules:
   - FP64: end states bit-identical to the CPU oracle on meshes with level
     jumps, periodic faces, per-cell albedo and g, two observers and several
     bands (one empty, the first ones on the persistent kernel in the same
-    launch), for few and many passes and several task sizes; J and images
-    equal to the order of FP64 sums;
+    launch), for few and many passes, several task sizes and packets sent
+    back to the queue every few steps; J and images equal to the order of
+    FP64 sums;
   - FP32: end states identical to the persistent mixed-precision kernel's,
     J to the FP32 rounding of its register accumulator;
   - a captured graph replays the same sequence.
@@ -26,9 +27,10 @@ pytestmark = pytest.mark.gpu
 PERSISTENT = dict(kind=_abi.KP_SCHED_PERSISTENT, block_threads=256, blocks_per_cu=4, chunk=64)
 
 
-def _bq(passes=0, chunk=0, from_band=0, bpc=4):
+def _bq(passes=0, chunk=0, from_band=0, bpc=4, yld=0):
     return dict(kind=_abi.KP_SCHED_BLOCKQ, block_threads=256, blocks_per_cu=bpc, chunk=64,
-                queue_from_band=from_band, queue_passes=passes, queue_chunk=chunk)
+                queue_from_band=from_band, queue_passes=passes, queue_chunk=chunk,
+                queue_yield=yld)
 
 
 def _amr_workload(n_packets, precision=_abi.KP_F64, bands=None):
@@ -94,12 +96,14 @@ def _close(a, b, rtol):
     np.testing.assert_allclose(a, b, rtol=rtol, atol=rtol * max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("passes,chunk", [(0, 0), (2, 64), (64, 256), (1, 4096)],
-                         ids=["default", "few_passes", "many_passes", "one_pass_big_tasks"])
-def test_f64_amr_bit_exact(oracle_lib, passes, chunk):
+@pytest.mark.parametrize("passes,chunk,yld", [(0, 0, 0), (2, 64, 0), (64, 256, 0),
+                                              (1, 4096, 0), (200, 256, 3)],
+                         ids=["default", "few_passes", "many_passes", "one_pass_big_tasks",
+                              "yield_every_3_steps"])
+def test_f64_amr_bit_exact(oracle_lib, passes, chunk, yld):
     w = _amr_workload(8000)
-    print(f"blockq amr f64 passes={passes} chunk={chunk}", flush=True)
-    got = _run(w, _bq(passes, chunk))
+    print(f"blockq amr f64 passes={passes} chunk={chunk} yield={yld}", flush=True)
+    got = _run(w, _bq(passes, chunk, yld=yld))
     ref = oracle_lib.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, w.run, end_state=True)
     _same_states(got["end_state"], ref["end_state"])
     assert np.array_equal(got["counters"], ref["counters"])
@@ -144,12 +148,12 @@ def test_f64_config1_bit_exact(oracle_lib):
     _close(got["image"], ref["image"], 1e-11)
 
 
-@pytest.mark.parametrize("passes", [3, 48])
-def test_f32_equals_persistent_mixed(passes):
+@pytest.mark.parametrize("passes,yld", [(3, 0), (48, 0), (100, 5)])
+def test_f32_equals_persistent_mixed(passes, yld):
     w = _amr_workload(6000, precision=_abi.KP_F32)
-    print(f"blockq f32 passes={passes}", flush=True)
+    print(f"blockq f32 passes={passes} yield={yld}", flush=True)
     a = _run(w, PERSISTENT)
-    b = _run(w, _bq(passes=passes, chunk=512))
+    b = _run(w, _bq(passes=passes, chunk=512, yld=yld))
     _same_states(b["end_state"], a["end_state"])
     assert np.array_equal(a["counters"], b["counters"])
     _close(b["image"], a["image"], 1e-9)
@@ -210,6 +214,8 @@ def test_schedule_rejections():
         eng.set_schedule(**{**_bq(), "queue_passes": -1})
     with pytest.raises(RuntimeError):
         eng.set_schedule(**{**_bq(), "queue_chunk": (1 << 20) + 1})
+    with pytest.raises(RuntimeError):
+        eng.set_schedule(**{**_bq(), "queue_yield": -2})
     big = Mesh(root_blocks=(1, 1, 1), n_cell=32).finalize()
     e2 = Engine(big)
     with pytest.raises(RuntimeError, match="4096"):
