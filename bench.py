@@ -48,6 +48,21 @@ def cpu_model():
     return "unknown CPU"
 
 
+def cpu_share():
+    """Threads of the CPU baseline: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS where the host sets it (the CPU share of one GPU on a
+    shared multi-GPU host)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def cpu_share_note():
+    return (f"{len(os.sched_getaffinity(0))} CPUs in this process's affinity set, "
+            f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}, "
+            f"{os.cpu_count()} CPUs on the host")
+
+
 def progress(msg):
     """One line per finished stage on stderr (stdout carries only the JSON)."""
     print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
@@ -71,6 +86,13 @@ def parse():
     p.add_argument("--large-packets", type=float, default=1e9)
     p.add_argument("--fd-cells", type=int, default=64,
                    help="cells of the per-cell finite-difference check of the adjoint (0: off)")
+    p.add_argument("--fd-packets", type=float, default=1e7,
+                   help="packets of the per-cell finite differences (split into --fd-batches)")
+    p.add_argument("--fd-batches", type=int, default=8)
+    p.add_argument("--fd-only", action="store_true",
+                   help="run only the finite-difference check (prints its JSON)")
+    p.add_argument("--skip-errors", action="store_true",
+                   help="skip the per-tally error study of the main workload")
     p.add_argument("--batched-scale", type=float, default=1.0,
                    help="scale of the per-band packet counts of the 64-band workload")
     p.add_argument("--skip-sph", action="store_true")
@@ -261,18 +283,22 @@ def batched(steps, scale):
 
 
 def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=1e-2,
-               n_dir_packets=40_000_000):
-    """Per-cell check of the FP64 path-replay gradient d loss / d log rho on the
-    adjoint workload against common-random-number central differences
-    (eps in log rho).  Cells are drawn with probability proportional to
-    |gradient| (cells no packet reaches have no measurable difference).
-    The packets are split into n_batches id ranges, each an estimate of the
-    same images (source weight n_batches); per batch the FD of that batch's
-    loss is compared with that batch's adjoint, giving mean and standard
-    error over batches.  The directional derivatives use their own, larger
-    packet set (n_dir_packets, same batching): the FD of a batch's loss and
-    its path-replay gradient differ per realisation by Monte Carlo noise,
-    which only more packets reduce."""
+               n_dir_packets=40_000_000, precision=None):
+    """Per-cell check of the path-replay gradients of the adjoint workload's
+    loss with respect to log rho, albedo and g against common-random-number
+    central differences (eps in log rho, albedo and g).
+
+    Cells: n_cells drawn uniformly at random (seeded) from the cells where
+    all three gradients are nonzero (cells where packets interact).  The n
+    packets are split into n_batches id ranges, each an estimate of the same
+    images (source weight n_batches); per batch the FD of that batch's loss
+    is paired with that batch's adjoint, giving the mean difference and its
+    standard error over batches.  Reported per parameter: the fraction of
+    cells within 2 %, within max(2 %, 3 standard errors), the median
+    standard error relative to |FD|, and the 2 % fraction among the cells
+    whose relative standard error is below 1 %.  The directional
+    derivatives along coherent fields use their own, larger packet set
+    (n_dir_packets, same batching)."""
     import numpy as np
     import torch
 
@@ -282,7 +308,8 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
     aw = configs.config4(n_packets=n)
     w = aw.base
     eng = Engine(w.mesh)
-    run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+    prec = _abi.KP_F64 if precision is None else precision
+    run = RunSpec(**{**w.run.__dict__, "precision": prec})
     nb = n // n_batches
     eng.set_media(aw.kappa_obs, w.albedo, w.g)
     obs_run = RunSpec(**{**run.__dict__, "packet_offset": 1 << 40})
@@ -302,55 +329,78 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
         res = out["image"][..., 1:] - q_obs
         return float((res * res).sum()), out
 
-    adj = []
+    keys = ("d_logrho", "d_albedo", "d_g")
+    adj = {k: [] for k in keys}
     eng.set_media(w.kappa, w.albedo, w.g)
     for b in range(n_batches):
         L0, out = loss(b)
         wgt = torch.zeros_like(out["image"])
         wgt[..., 1:] = 2.0 * (out["image"][..., 1:] - q_obs)
         g = eng.adjoint(w.source, batch_run(b), wgt, weight=float(n_batches))
-        adj.append(g["d_logrho"].cpu().numpy())
-    adj = np.array(adj)
-    mean_adj = adj.mean(axis=0)
-    p = np.abs(mean_adj)
+        for k in keys:
+            adj[k].append(g[k].cpu().numpy())
+    adj = {k: np.array(v) for k, v in adj.items()}
+    nz = np.ones(w.mesh.n_cells, dtype=bool)
+    for k in keys:
+        nz &= adj[k].mean(axis=0) != 0.0
     rng = np.random.default_rng(seed)
-    cells = rng.choice(len(p), size=n_cells, replace=False, p=p / p.sum())
-    def cell_fd(e):
-        fd = np.zeros((n_batches, n_cells))
-        for k, c in enumerate(cells):
-            # the exact log-rho step after rounding kappa_rho to FP32
+    cells = rng.choice(np.nonzero(nz)[0], size=n_cells, replace=False)
+
+    def perturbed(param, c, sgn, e):
+        """medium with cell c's parameter stepped by sgn*e, and the exact step
+        after rounding to FP32"""
+        kap, alb, gg = w.kappa.copy(), w.albedo.copy(), w.g.copy()
+        if param == "d_logrho":
             kp = np.float32(w.kappa[c] * np.exp(e))
             km = np.float32(w.kappa[c] * np.exp(-e))
+            kap[c] = kp if sgn > 0 else km
             step = float(np.log(np.float64(kp) / np.float64(km)))
-            for sgn, kv in ((1.0, kp), (-1.0, km)):
-                kap = w.kappa.copy()
-                kap[c] = kv
-                eng.set_media(kap, w.albedo, w.g)
+        else:
+            base = w.albedo if param == "d_albedo" else w.g
+            vp, vm = np.float32(base[c] + e), np.float32(base[c] - e)
+            (alb if param == "d_albedo" else gg)[c] = vp if sgn > 0 else vm
+            step = float(np.float64(vp) - np.float64(vm))
+        return kap, alb, gg, step
+
+    def cell_fd(param, e):
+        fd = np.zeros((n_batches, n_cells))
+        for k, c in enumerate(cells):
+            for sgn in (1.0, -1.0):
+                kap, alb, gg, step = perturbed(param, c, sgn, e)
+                eng.set_media(kap, alb, gg)
                 for b in range(n_batches):
                     fd[b, k] += sgn * loss(b)[0] / step
             if (k + 1) % 16 == 0:
-                progress(f"finite differences (eps {e:g}): {k + 1} of {n_cells} cells")
+                progress(f"finite differences {param} (eps {e:g}): {k + 1} of {n_cells} cells")
         return fd
 
-    def compare(fd):
-        a = adj[:, cells]
+    def compare(param, fd):
+        a = adj[param][:, cells]
         m_fd, m_ad = fd.mean(axis=0), a.mean(axis=0)
         se = (a - fd).std(axis=0, ddof=1) / np.sqrt(n_batches)
         rel = np.abs(m_ad - m_fd) / np.maximum(np.abs(m_fd), 1e-300)
+        se_rel = se / np.maximum(np.abs(m_fd), 1e-300)
+        quiet = se_rel < 0.01
         return {"median_rel_diff": float(np.median(rel)),
                 "frac_within_2pct": float(np.mean(rel <= 0.02)),
                 "frac_within_2pct_or_3se": float(np.mean((rel <= 0.02) |
                                                          (np.abs(m_ad - m_fd) <= 3 * se))),
+                "frac_within_3se": float(np.mean(np.abs(m_ad - m_fd) <= 3 * se)),
+                "median_se_rel": float(np.median(se_rel)),
+                "cells_se_below_1pct": int(quiet.sum()),
+                "frac_within_2pct_of_cells_se_below_1pct":
+                    float(np.mean(rel[quiet] <= 0.02)) if quiet.any() else None,
                 "worst_rel_diff": float(rel.max())}
 
     t0 = time.perf_counter()
-    per_cell = {f"eps_{eps:g}": compare(cell_fd(eps)), f"eps_{eps_dir:g}": compare(cell_fd(eps_dir))}
+    per_cell = {f"{p}_eps_{eps:g}": compare(p, cell_fd(p, eps)) for p in keys}
+    eng.set_media(w.kappa, w.albedo, w.g)
     # directional derivatives d/dt L(log rho + t v) along coherent fields v:
     # toward the observed medium, a uniform rescaling, and smooth random
     # k^-11/3 fields; FD vs v . grad
     xc = w.mesh.cell_centers()
     dirs = [np.log(aw.kappa_obs.astype(np.float64) / np.maximum(w.kappa, 1e-30)),
-            np.ones(len(p))]
+            np.ones(w.mesh.n_cells)]
     for sd in range(6):
         dirs.append(configs._sample_field(configs.gaussian_random_field(64, 100 + sd), xc))
     n_dir = len(dirs)
@@ -383,7 +433,9 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
     dm_fd, dm_ad = dd_fd.mean(axis=0), dd_ad.mean(axis=0)
     d_se = (dd_ad - dd_fd).std(axis=0, ddof=1) / np.sqrt(n_batches)
     d_rel = np.abs(dm_ad - dm_fd) / np.abs(dm_fd)
-    return {"workload": w.name, "packets": n, "batches": n_batches, "cells": n_cells,
+    return {"workload": w.name, "precision": "f64" if prec == _abi.KP_F64 else "mixed",
+            "packets": n, "batches": n_batches, "cells": n_cells,
+            "cells_drawn": "uniform among cells with all three gradients nonzero",
             "fd_seconds": t_fd, "per_cell": per_cell,
             "directional": {"directions": n_dir, "eps": eps_dir, "packets": n_dir_packets,
                             "rel_diff": [float(x) for x in d_rel],
@@ -391,7 +443,7 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
                             "max_rel_diff": float(d_rel.max())}}
 
 
-def large(packets=1_000_000_000, chunk=100_000_000):
+def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
     """Config 5: 1e9 packets on the ~28,000-block disk mesh, streamed as device
     chunks of `chunk` packets replayed from one hipGraph (autotuned schedule,
     mixed precision, packet order re-checked on one chunk); FP64 on one chunk
@@ -448,13 +500,25 @@ def large(packets=1_000_000_000, chunk=100_000_000):
     out = eng.alloc_outputs(w.run)
     g = eng.graph(w.source, w.run, out, n_chunks=n_chunks)
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    g.launch()
+    ev0.record(stream)
+    g.launch(stream)
+    ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kern = ev0.elapsed_time(ev1) / 1e3
     cnt = out["counters"].cpu().numpy().astype(np.float64)
     res["mixed"] = {"seconds": wall, "value": float((cnt[0] + cnt[1]) / wall),
                     "packets_done": float(cnt[7]), "crossings": float(cnt[0] + cnt[1])}
+    b_alg = algorithmic_bytes(cnt, len(w.run.incl_deg))
+    res["roofline"] = {"bound": "hbm", "achieved": b_alg / kern / 1e9, "peak": HBM_PEAK / 1e9,
+                       "unit": "GB/s", "frac": b_alg / kern / HBM_PEAK, "traffic": None,
+                       "alg_bytes": b_alg, "graph_seconds_hip_events": kern,
+                       "note": "algorithmic bytes of all chunks over the HIP-event time of "
+                               "the graph replay on its stream"}
     img32 = out["image"].cpu().numpy()
     del g, out
     progress(f"large: {packets:.3g} packets in {wall:.1f} s")
@@ -468,6 +532,10 @@ def large(packets=1_000_000_000, chunk=100_000_000):
     res["gradient_pass"] = {"seconds": time.perf_counter() - t0, "packets": chunk,
                             "d_logrho_norm": float(torch.linalg.vector_norm(gr["d_logrho"])),
                             "jlin": float(gr["jlin"][0])}
+    del gr
+    if cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_baseline(w, cpu_seconds)
+        progress("large: CPU baseline done")
     return res
 
 
@@ -590,7 +658,7 @@ def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
     if not no_cpu:
         import oracle
 
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores = cpu_share()
         npk, dt = 20000, 0.0
         while True:
             r = RunSpec(**{**w.run.__dict__, "bands": [type(w.run.bands[0])(
@@ -604,9 +672,110 @@ def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
         res["cpu_baseline"] = {
             "value": float(o["counters"][_abi.KP_C_CROSS_MAIN] / dt), "unit": "cell-crossings/s",
             "cores": cores, "kind": "port",
+            "host_cpus": os.cpu_count(),
             "sample": f"{npk} packets of the same workload (FP64 spherical oracle, "
-                      f"pthreads), {dt:.1f} s on {cores} threads of {cpu_model()}"}
+                      f"pthreads), {dt:.1f} s on {cores} threads ({cpu_share_note()}) of "
+                      f"{cpu_model()}"}
     del out, eng
+    return res
+
+
+def cpu_baseline(w, cpu_seconds, max_packets=5e7):
+    """The FP64 CPU oracle on a sample of workload w sized to about
+    cpu_seconds of work (pilot runs first): cell-crossings/s."""
+    import numpy as np
+
+    import oracle
+    from kratos_polrad_amd.engine import RunSpec
+
+    from kratos_polrad_amd import _abi
+
+    cores = cpu_share()
+    run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
+    b0 = run.bands[0]
+
+    def cpu_run(npk):
+        run.bands = [type(b0)(**{**b0.__dict__, "n_packets": npk})]
+        o = oracle.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, run, n_threads=cores)
+        return o["transport_seconds"], o["counters"].astype(np.float64)
+
+    # the transport phase is timed: the per-thread FP64 tally copies (one per
+    # thread, 1 GB each on the 1.2e8-cell disk) and their reduction are not
+    npk = 1000
+    dt, c = cpu_run(npk)
+    while dt < 1.0 and npk < max_packets:
+        npk *= 8
+        dt, c = cpu_run(npk)
+    if dt < 0.5 * cpu_seconds:
+        npk = int(min(npk * cpu_seconds / max(dt, 1e-3), max_packets))
+        dt, c = cpu_run(npk)
+    return {"value": float((c[0] + c[1]) / dt), "unit": "cell-crossings/s", "cores": cores,
+            "kind": "port", "host_cpus": os.cpu_count(),
+            "sample": f"{npk} packets of the same workload (FP64 oracle, pthreads), transport "
+                      f"phase {dt:.1f} s on {cores} threads ({cpu_share_note()}) of "
+                      f"{cpu_model()}"}
+
+
+def tally_errors(eng, w, n_batches=8, per=10_000_000):
+    """Error of each tally of the mixed path against FP64.  Per batch of
+    `per` packets: the mixed path, the FP64 path on the same packets and an
+    FP64 run on other packet ids; J and the I, Q, U images summed on the
+    device.  Reported: relative L2 differences of the batch means (J and
+    each Stokes image), and per cell of J the fraction of cells (those every
+    batch of every run reached) whose mixed mean lies within 3 combined
+    standard errors of the independent FP64 mean, beside the same fraction
+    for the FP64 run against the independent one."""
+    import numpy as np
+    import torch
+
+    from kratos_polrad_amd import _abi
+    from kratos_polrad_amd.engine import RunSpec
+
+    b0 = w.run.bands[0]
+    base = RunSpec(**{**w.run.__dict__, "bands": [type(b0)(**{**b0.__dict__,
+                                                            "n_packets": per})]})
+    stats = {}
+    for name, prec, off in (("mixed", _abi.KP_F32, 0), ("fp64", _abi.KP_F64, 0),
+                            ("fp64_independent", _abi.KP_F64, 1 << 40)):
+        s1 = s2 = hit = img = None
+        for b in range(n_batches):
+            r = RunSpec(**{**base.__dict__, "precision": prec, "packet_offset": off + b * per})
+            out = eng.alloc_outputs(r, counters=False)
+            eng.forward(w.source, r, out)
+            J = out["J"]
+            s1 = J.clone() if s1 is None else s1 + J
+            s2 = J * J if s2 is None else s2 + J * J
+            hit = (J > 0) if hit is None else hit & (J > 0)
+            img = out["image"].clone() if img is None else img + out["image"]
+        torch.cuda.synchronize()
+        m = s1 / n_batches
+        se = ((s2 / n_batches - m * m).clamp(min=0) / (n_batches - 1)).sqrt()
+        stats[name] = (m, se, hit, img / n_batches)
+        progress(f"tally errors: {name} done")
+    m32, e32, h32, i32 = stats["mixed"]
+    m64, e64, h64, i64 = stats["fp64"]
+    mi, ei, hi, ii = stats["fp64_independent"]
+    sel = h32 & h64 & hi
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    within = lambda a, ea, b, eb: float(((a - b).abs() <= 3 * (ea * ea + eb * eb).sqrt())[sel]
+                                        .double().mean())
+    res = {"batches": n_batches, "packets_per_batch": per,
+           "J": {"rel_l2_mixed_vs_independent_fp64": rel(m32, mi),
+                 "rel_l2_fp64_vs_independent_fp64": rel(m64, mi),
+                 "rel_l2_mixed_vs_fp64_same_packets": rel(m32, m64),
+                 "cells_compared": int(sel.sum()),
+                 "frac_cells_mixed_within_3se_of_independent_fp64": within(m32, e32, mi, ei),
+                 "frac_cells_fp64_within_3se_of_independent_fp64": within(m64, e64, mi, ei)}}
+    inorm = torch.linalg.vector_norm(ii[..., 0])
+    for k, name in enumerate("IQU"):
+        res[name] = {
+            "rel_l2_mixed_vs_independent_fp64": float(torch.linalg.vector_norm(
+                i32[..., k] - ii[..., k]) / inorm),
+            "rel_l2_fp64_vs_independent_fp64": float(torch.linalg.vector_norm(
+                i64[..., k] - ii[..., k]) / inorm),
+            "rel_l2_mixed_vs_fp64_same_packets": float(torch.linalg.vector_norm(
+                i32[..., k] - i64[..., k]) / inorm)}
+    res["note"] = "images relative to the independent FP64 run's I norm"
     return res
 
 
@@ -620,6 +789,11 @@ def main():
 
     from kratos_polrad_amd import _abi, configs
     from kratos_polrad_amd.engine import Engine, RunSpec
+
+    if args.fd_only:
+        print(json.dumps(adjoint_fd(int(args.fd_packets), args.fd_cells,
+                                    n_batches=args.fd_batches)))
+        return
 
     if args.copy_calibration:
         nbytes = 512 << 20
@@ -749,6 +923,9 @@ def main():
 
     if not args.skip_f64:
         progress("FP64 steps done")
+        if not args.skip_errors:
+            res["tally_error"] = tally_errors(eng, w)
+            progress("per-tally errors done")
     if not args.skip_batched:
         res["batched"] = batched(args.steps, args.batched_scale)
         progress("64-band batched run done")
@@ -757,7 +934,8 @@ def main():
         res["adjoint"] = adjoint_ratio(int(args.adjoint_packets))
         progress("adjoint timing done")
         if args.fd_cells > 0:
-            res["adjoint"]["fd_check"] = adjoint_fd(int(args.adjoint_packets), args.fd_cells)
+            res["adjoint"]["fd_check"] = adjoint_fd(int(args.fd_packets), args.fd_cells,
+                                                    n_batches=args.fd_batches)
             progress("finite-difference check done")
 
     if not args.skip_compaction:
@@ -765,7 +943,8 @@ def main():
         progress("compaction done")
 
     if not args.skip_large:
-        res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000))
+        res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000),
+                             0.0 if args.no_cpu else args.cpu_seconds / 2)
         progress("large run done")
 
     if not args.skip_sph:
@@ -774,33 +953,7 @@ def main():
         progress("spherical run done")
 
     if not args.no_cpu:
-        import oracle
-
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        run = RunSpec(**{**w.run.__dict__, "precision": _abi.KP_F64})
-        b0 = run.bands[0]
-
-        def cpu_run(npk):
-            run.bands = [type(b0)(**{**b0.__dict__, "n_packets": npk})]
-            t0 = time.perf_counter()
-            o = oracle.forward(w.mesh, w.kappa, w.albedo, w.g, w.source, run, n_threads=cores)
-            return time.perf_counter() - t0, o["counters"].astype(np.float64)
-
-        # pilots to size a sample of about --cpu-seconds of CPU work
-        npk = 2000
-        dt, c = cpu_run(npk)
-        while dt < 1.0 and npk < 5e7:
-            npk *= 8
-            dt, c = cpu_run(npk)
-        if dt < 0.5 * args.cpu_seconds:
-            npk = int(min(npk * args.cpu_seconds / max(dt, 1e-3), 5e7))
-            dt, c = cpu_run(npk)
-        n_cpu = npk
-        res["cpu_baseline"] = {
-            "value": float((c[0] + c[1]) / dt), "unit": "cell-crossings/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{n_cpu} packets of the same workload (FP64 oracle, pthreads), "
-                      f"{dt:.1f} s on {cores} threads of {cpu_model()}"}
+        res["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(res))
 
 

@@ -77,6 +77,8 @@ def lib():
         L.or_sph_forward.restype = C.c_int
         L.or_sph_forward.argtypes = [C.POINTER(_abi.MeshDesc), C.POINTER(_abi.Source),
                                      C.POINTER(_abi.Run), P, P, P, C.c_int]
+        L.or_last_transport_seconds.restype = C.c_double
+        L.or_last_transport_seconds.argtypes = []
         L.or_sph_trace.restype = C.c_int
         L.or_sph_trace.argtypes = L.or_trace.argtypes
         L.or_sph_locate.restype = C.c_int
@@ -118,6 +120,7 @@ def forward(mesh, kappa, albedo, g, source_x, run_spec, weight=1.0, J=True, imag
     del keep
     if rc != 0:
         raise RuntimeError(f"or_forward failed: {rc}")
+    out["transport_seconds"] = float(lib().or_last_transport_seconds())
     return out
 
 

@@ -12,6 +12,7 @@
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -638,6 +639,11 @@ static void *worker(void *arg)
     return NULL;
 }
 
+/* wall time of the last or_forward's transport phase (threads started to
+ * threads joined: no tally allocation or per-thread reduction) */
+static double g_transport_seconds = 0.0;
+double or_last_transport_seconds(void) { return g_transport_seconds; }
+
 int or_forward(const kp_mesh_desc *m, const float *kappa, const float *albedo, const float *g,
                const kp_source *src, const kp_run *run, double *J, double *image,
                uint64_t *counters, double *end_state, int n_threads)
@@ -700,11 +706,15 @@ int or_forward(const kp_mesh_desc *m, const float *kappa, const float *albedo, c
         }
     }
     if (rc == 0) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
         for (int k = 1; k < n_threads; ++k)
             pthread_create(&tid[k], NULL, worker, &th[k]);
         worker(&th[0]);
         for (int k = 1; k < n_threads; ++k)
             pthread_join(tid[k], NULL);
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        g_transport_seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
         for (int k = 1; k < n_threads; ++k) {
             if (J)
                 for (size_t q = 0; q < nJ; ++q)

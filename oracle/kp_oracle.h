@@ -46,6 +46,11 @@ int or_forward(const kp_mesh_desc *mesh, const float *kappa, const float *albedo
                const float *g, const kp_source *src, const kp_run *run, double *J,
                double *image, uint64_t *counters, double *end_state, int n_threads);
 
+/* Wall time (seconds) of the transport phase of the last or_forward call:
+ * from starting its threads to joining them, without the allocation and the
+ * reduction of the per-thread tallies. */
+double or_last_transport_seconds(void);
+
 /* Adjoint by path replay of the linear image functional
  * Jlin = sum_pix dL_dimage[pix] . image[pix] (image layout, I,Q,U):
  * accumulates dJlin/dlog(rho_c), dJlin/d(albedo_c), dJlin/d(g_c) per cell
