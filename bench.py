@@ -91,13 +91,35 @@ def parse():
     p.add_argument("--fd-batches", type=int, default=8)
     p.add_argument("--fd-only", action="store_true",
                    help="run only the finite-difference check (prints its JSON)")
+    p.add_argument("--fd-eps-logrho", type=float, default=0.0,
+                   help="log rho step of the per-cell finite differences (0: 1e-3)")
+    p.add_argument("--fd-params", default="d_logrho,d_albedo,d_g",
+                   help="comma-separated gradients of the --fd-only check")
+    p.add_argument("--fd-no-directional", action="store_true",
+                   help="skip the directional derivatives of the --fd-only check")
     p.add_argument("--skip-errors", action="store_true",
                    help="skip the per-tally error study of the main workload")
     p.add_argument("--batched-scale", type=float, default=1.0,
                    help="scale of the per-band packet counts of the 64-band workload")
+    p.add_argument("--batched-only", action="store_true",
+                   help="run only the 64-band batched section (prints its JSON)")
+    p.add_argument("--batched-bands", default="0:64",
+                   help="band range LO:HI of the 64-band workload to run")
+    p.add_argument("--batched-kind", default="screen",
+                   choices=["screen", "persistent", "block_queue"],
+                   help="schedule of the batched run: the screened fastest, or the tuned "
+                        "persistent kernel, or every band in the block queue")
+    p.add_argument("--batched-fixed", action="store_true",
+                   help="with --batched-only and a --batched-kind other than screen: the "
+                        "default schedule of that kind, mixed precision only, no autotuner")
     p.add_argument("--skip-sph", action="store_true")
     p.add_argument("--sph-packets", type=float, default=1e8,
                    help="packets per step of the spherical-mesh streaming workload")
+    p.add_argument("--sph-only", action="store_true",
+                   help="run only the spherical-mesh section (prints its JSON)")
+    p.add_argument("--sph-coherent", type=int, default=-1, choices=(-1, 0, 1),
+                   help="spherical packet order without the autotuner (0 slot, 1 emission "
+                        "direction; -1: autotuned)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 512 MiB device-to-device copy (3 times) so "
                         "rocprofv3 counters can be scaled by a kernel of known traffic")
@@ -185,12 +207,17 @@ def adjoint_ratio(n):
                                  for k, v in gr32.items() if k != "jlin"}}
 
 
-def batched(steps, scale):
+def batched(steps, scale, bands="0:64", kind="screen", fixed=False):
     """64-band batched forward (config 3): all bands in one launch captured
     into a hipGraph, each precision on its own fastest schedule (the
     autotuner's pick, screened at full size against its other packet order
     and against the persistent kernel or the block queue for the optically
-    thin bands); mixed precision vs the FP64 path on the same packets."""
+    thin bands); mixed precision vs the FP64 path on the same packets.
+
+    fixed (for counter profiles): no autotuner and no screening; the mixed
+    path alone on the engine's default schedule of the given kind (the block
+    queue from the first band), one warm-up launch and `steps` timed
+    launches of the same packets."""
     import numpy as np
     import torch
 
@@ -198,6 +225,11 @@ def batched(steps, scale):
     from kratos_polrad_amd.engine import Engine, RunSpec
 
     w = configs.config3(precision=_abi.KP_F32, scale_packets=scale)
+    lo, hi = (int(x) for x in bands.split(":"))
+    if (lo, hi) != (0, len(w.run.bands)):
+        w = configs.Workload(w.name + f"_bands_{lo}_{hi}", w.mesh, w.kappa, w.albedo, w.g,
+                             w.source, RunSpec(**{**w.run.__dict__,
+                                                  "bands": w.run.bands[lo:hi]}))
     eng = Engine(w.mesh)
     eng.set_media(w.kappa, w.albedo, w.g)
     res = {"workload": w.name, "bands": len(w.run.bands), "packets_per_step": w.run.total_packets,
@@ -237,12 +269,27 @@ def batched(steps, scale):
         # two replays each, and the fastest is timed
         run = RunSpec(**{**w.run.__dict__, "precision": prec})
         t0 = time.perf_counter()
+        if fixed:
+            sc = {**eng.get_schedule(), "queue_from_band": 0,
+                  "kind": (_abi.KP_SCHED_BLOCKQ if kind == "block_queue"
+                           else _abi.KP_SCHED_PERSISTENT)}
+            res["schedule_mixed"] = sc
+            wall, cnt, img = run_graph(prec, sc, steps)
+            res["launches"] = steps + 1
+            progress(f"batched (fixed {kind}): {1e3 * wall / steps:.0f} ms per step")
+            return wall, cnt, img
         tuned = eng.autotune(w.source, run, sample_packets=10_000_000)
         res["autotune_s"] += time.perf_counter() - t0
         progress(f"batched: precision {prec} tuned in {time.perf_counter() - t0:.1f} s "
                  f"(kind {tuned['kind']}, queue from band {tuned['queue_from_band']})")
+        if kind == "persistent":
+            tuned = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
+        elif kind == "block_queue":
+            tuned = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0}
         cands = {"tuned": tuned, "tuned_other_order": {**tuned, "coherent": 1 - tuned["coherent"]}}
-        if tuned["kind"] == _abi.KP_SCHED_BLOCKQ:
+        if kind != "screen":
+            pass
+        elif tuned["kind"] == _abi.KP_SCHED_BLOCKQ:
             cands["persistent"] = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
         else:
             for k in sorted({0, first_thin}):
@@ -265,6 +312,10 @@ def batched(steps, scale):
     wall32, cnt32, img32 = best_schedule(_abi.KP_F32)
     res["value"] = float((cnt32[0] + cnt32[1]) / wall32)
     res["ms_per_step"] = 1e3 * wall32 / steps
+    res["crossings_per_step_mixed"] = {"main": float(cnt32[0] / steps),
+                                       "peel": float(cnt32[1] / steps)}
+    if fixed:
+        return res
     wall64, cnt64, img64 = best_schedule(_abi.KP_F64)
     res["fp64_value"] = float((cnt64[0] + cnt64[1]) / wall64)
     res["speedup_mixed_vs_fp64"] = res["value"] / res["fp64_value"]
@@ -283,7 +334,8 @@ def batched(steps, scale):
 
 
 def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=1e-2,
-               n_dir_packets=40_000_000, precision=None):
+               n_dir_packets=40_000_000, precision=None, eps_logrho=None,
+               params=("d_logrho", "d_albedo", "d_g"), directional=True):
     """Per-cell check of the path-replay gradients of the adjoint workload's
     loss with respect to log rho, albedo and g against common-random-number
     central differences (eps in log rho, albedo and g).
@@ -295,10 +347,12 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
     is paired with that batch's adjoint, giving the mean difference and its
     standard error over batches.  Reported per parameter: the fraction of
     cells within 2 %, within max(2 %, 3 standard errors), the median
-    standard error relative to |FD|, and the 2 % fraction among the cells
-    whose relative standard error is below 1 %.  The directional
-    derivatives along coherent fields use their own, larger packet set
-    (n_dir_packets, same batching)."""
+    standard error, and the 2 % fraction among the cells whose relative
+    standard error is below 1 %; differences and errors are relative to
+    max(|adjoint|, |FD|), so a cell whose FD mean is noise near zero does not
+    divide by it.  eps_logrho (default eps) is the log rho step.  The
+    directional derivatives along coherent fields use their own, larger
+    packet set (n_dir_packets, same batching)."""
     import numpy as np
     import torch
 
@@ -378,8 +432,9 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
         a = adj[param][:, cells]
         m_fd, m_ad = fd.mean(axis=0), a.mean(axis=0)
         se = (a - fd).std(axis=0, ddof=1) / np.sqrt(n_batches)
-        rel = np.abs(m_ad - m_fd) / np.maximum(np.abs(m_fd), 1e-300)
-        se_rel = se / np.maximum(np.abs(m_fd), 1e-300)
+        scale = np.maximum(np.maximum(np.abs(m_fd), np.abs(m_ad)), 1e-300)
+        rel = np.abs(m_ad - m_fd) / scale
+        se_rel = se / scale
         quiet = se_rel < 0.01
         return {"median_rel_diff": float(np.median(rel)),
                 "frac_within_2pct": float(np.mean(rel <= 0.02)),
@@ -387,22 +442,43 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
                                                          (np.abs(m_ad - m_fd) <= 3 * se))),
                 "frac_within_3se": float(np.mean(np.abs(m_ad - m_fd) <= 3 * se)),
                 "median_se_rel": float(np.median(se_rel)),
+                "median_se_abs": float(np.median(se)),
+                "median_abs_adjoint": float(np.median(np.abs(m_ad))),
+                "median_abs_fd": float(np.median(np.abs(m_fd))),
+                "adjoint_batch_se_abs_median": float(np.median(
+                    a.std(axis=0, ddof=1) / np.sqrt(n_batches))),
                 "cells_se_below_1pct": int(quiet.sum()),
                 "frac_within_2pct_of_cells_se_below_1pct":
                     float(np.mean(rel[quiet] <= 0.02)) if quiet.any() else None,
                 "worst_rel_diff": float(rel.max())}
 
     t0 = time.perf_counter()
-    per_cell = {f"{p}_eps_{eps:g}": compare(p, cell_fd(p, eps)) for p in keys}
+    per_cell = {}
+    for p in params:
+        e = eps_logrho if (p == "d_logrho" and eps_logrho) else eps
+        per_cell[f"{p}_eps_{e:g}"] = compare(p, cell_fd(p, e))
     eng.set_media(w.kappa, w.albedo, w.g)
+    if not directional:
+        return {"workload": w.name, "precision": "f64" if prec == _abi.KP_F64 else "mixed",
+                "packets": n, "batches": n_batches, "cells": n_cells,
+                "cells_drawn": "uniform among cells with all three gradients nonzero",
+                "fd_seconds": time.perf_counter() - t0, "per_cell": per_cell}
     # directional derivatives d/dt L(log rho + t v) along coherent fields v:
     # toward the observed medium, a uniform rescaling, and smooth random
     # k^-11/3 fields; FD vs v . grad
     xc = w.mesh.cell_centers()
     dirs = [np.log(aw.kappa_obs.astype(np.float64) / np.maximum(w.kappa, 1e-30)),
             np.ones(w.mesh.n_cells)]
+    names = ["toward_observed", "uniform"]
     for sd in range(6):
         dirs.append(configs._sample_field(configs.gaussian_random_field(64, 100 + sd), xc))
+        names.append(f"random_field_{sd}")
+    # balls of cells around two of the drawn cells: the step from one cell
+    # to the coherent fields
+    for r in (0.02, 0.05, 0.1):
+        for c in cells[:2]:
+            dirs.append((np.sum((xc - xc[c]) ** 2, axis=1) <= r * r).astype(np.float64))
+            names.append(f"ball_r{r:g}_cell{int(c)}_{int(dirs[-1].sum())}cells")
     n_dir = len(dirs)
     dd_fd = np.zeros((n_batches, n_dir))
     dd_ad = np.zeros((n_batches, n_dir))
@@ -438,6 +514,7 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
             "cells_drawn": "uniform among cells with all three gradients nonzero",
             "fd_seconds": t_fd, "per_cell": per_cell,
             "directional": {"directions": n_dir, "eps": eps_dir, "packets": n_dir_packets,
+                            "names": names,
                             "rel_diff": [float(x) for x in d_rel],
                             "diff_over_se": [float(x) for x in np.abs(dm_ad - dm_fd) / d_se],
                             "max_rel_diff": float(d_rel.max())}}
@@ -606,11 +683,13 @@ def compaction(n=100_000_000, unit=64, n_bins=2234, p_keep=0.7):
     return res
 
 
-def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
+def spherical(steps, warmup, packets, cpu_seconds, no_cpu, coherent=-1):
     """The reference's spherical photon (photon_sph.h) on the spherical shell
     workload: pure streaming with the path-length tally, FP64, bit-exact with
     the CPU restatement.  A step is one kp_forward launch over `packets`
-    packets; value = packet-path crossings / wall time."""
+    packets; value = packet-path crossings / wall time.  coherent >= 0 skips
+    the autotuner and runs the engine's default spherical schedule with that
+    packet order (0: slot order, 1: emission-direction order)."""
     import numpy as np
     import torch
 
@@ -624,7 +703,11 @@ def spherical(steps, warmup, packets, cpu_seconds, no_cpu):
     res = {"workload": w.name, "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
            "levels": int(w.mesh.level.max()) + 1, "packets_per_step": n}
     t0 = time.perf_counter()
-    res["schedule"] = eng.autotune(w.source, w.run, sample_packets=10_000_000)
+    if coherent >= 0:
+        eng.set_schedule(**{**eng.get_schedule(), "coherent": int(coherent)})
+        res["schedule"] = eng.get_schedule()
+    else:
+        res["schedule"] = eng.autotune(w.source, w.run, sample_packets=10_000_000)
     res["autotune_seconds"] = time.perf_counter() - t0
     run = RunSpec(**w.run.__dict__)
     out = eng.alloc_outputs(run)
@@ -792,7 +875,22 @@ def main():
 
     if args.fd_only:
         print(json.dumps(adjoint_fd(int(args.fd_packets), args.fd_cells,
-                                    n_batches=args.fd_batches)))
+                                    n_batches=args.fd_batches,
+                                    eps_logrho=args.fd_eps_logrho or None,
+                                    params=tuple(q for q in args.fd_params.split(",") if q),
+                                    directional=not args.fd_no_directional)))
+        return
+
+    if args.sph_only:
+        print(json.dumps(spherical(args.steps, args.warmup, args.sph_packets, args.cpu_seconds,
+                                   args.no_cpu, args.sph_coherent)))
+        return
+
+    if args.batched_only:
+        if args.batched_fixed and args.batched_kind == "screen":
+            sys.exit("--batched-fixed needs --batched-kind persistent or block_queue")
+        print(json.dumps(batched(args.steps, args.batched_scale, args.batched_bands,
+                                 args.batched_kind, args.batched_fixed)))
         return
 
     if args.copy_calibration:
@@ -927,7 +1025,8 @@ def main():
             res["tally_error"] = tally_errors(eng, w)
             progress("per-tally errors done")
     if not args.skip_batched:
-        res["batched"] = batched(args.steps, args.batched_scale)
+        res["batched"] = batched(args.steps, args.batched_scale, args.batched_bands,
+                                 args.batched_kind)
         progress("64-band batched run done")
 
     if not args.skip_adjoint:

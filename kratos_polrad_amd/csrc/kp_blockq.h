@@ -303,9 +303,15 @@ enum : int { MARCH_END = 0, MARCH_OUT = 1 };
 // holds it in the new block), or after Q.yield steps and interactions in
 // this block (it goes back to the queue where it stands).  Otherwise kappa_rho from HBM and J by global
 // atomics, to the packet's end.  MARCH_END: L.status holds the end status.
+// J tile of a block in LDS: FP64 for the FP64 path; FP32 for the mixed path,
+// whose per-task sums (a few hundred contributions per cell at most) keep
+// FP32's 7 digits, far below the Monte Carlo noise, before the FP64 atomic
+// flush.  Half the LDS lets more workgroups share a CU.
+template <class Pol> using tile_t = typename Pol::real;
+
 template <class Pol, bool LOCAL>
 __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
-                     double *lj)
+                     tile_t<Pol> *lj)
 {
     using real = typename Pol::real;
     uint32_t work = 0; // steps and interactions in this pass (LOCAL)
@@ -350,15 +356,15 @@ __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const fl
                 seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
             }
             if (P.J) {
-                double v;
+                real v;
                 if constexpr (sizeof(real) == 8)
                     v = L.I * (seg / P.vol0[L.r.ib]);
                 else
-                    v = (double)(L.I * seg * L.r.iv);
+                    v = L.I * seg * L.r.iv;
                 if constexpr (LOCAL)
                     atomicAdd(&lj[cell], v);
                 else
-                    atomic_add_f64(j_at(P, (uint64_t)gc, L.band), v);
+                    atomic_add_f64(j_at(P, (uint64_t)gc, L.band), (double)v);
             }
             if (hit) {
                 Pol::advance(L.r, seg);

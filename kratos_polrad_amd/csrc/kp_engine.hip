@@ -520,12 +520,13 @@ __global__ void __launch_bounds__(256) k_bq_seg(const uint32_t *skey,
 // One pass of the block queue: tasks of `chunk` sorted records; per block of
 // a task the block's kappa_rho and a J tile in LDS, each packet marched to
 // the block's edge or its end, the tile added to J.
-template <class Pol>
-__global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams Q)
+template <class Pol, int WPS>
+__global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qparams Q)
 {
     using real = typename Pol::real;
-    double *lj = kp::kp_lds;
-    float *lk = reinterpret_cast<float *>(kp::kp_lds + P.n3);
+    using tile = kp::bq::tile_t<Pol>;
+    tile *lj = reinterpret_cast<tile *>(kp::kp_lds);
+    float *lk = reinterpret_cast<float *>(lj + P.n3);
     __shared__ uint32_t s_key;
     __shared__ uint64_t s_end;
     kp::lane_counters C;
@@ -553,7 +554,7 @@ __global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams 
             const float *kb = P.kappa + (uint64_t)ib * P.n3;
             for (int c = threadIdx.x; c < P.n3; c += blockDim.x) {
                 lk[c] = kb[c];
-                lj[c] = 0.0;
+                lj[c] = (tile)0;
             }
             __syncthreads();
             for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
@@ -568,7 +569,7 @@ __global__ void __launch_bounds__(256) k_bq_pass(kp::kparams P, kp::bq::qparams 
             if (P.J) {
                 const uint64_t g0 = (uint64_t)ib * P.n3;
                 for (int c = threadIdx.x; c < P.n3; c += blockDim.x) {
-                    const double v = lj[c];
+                    const double v = (double)lj[c];
                     if (v != 0.0)
                         kp::atomic_add_f64(kp::j_at(P, g0 + c, band), v);
                 }
@@ -1443,8 +1444,12 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
         Q.out = fb.bq_rec[nxt];
         Q.out_key = fb.bq_key[nxt];
         Q.n_out = fb.bq_cnt + nxt;
-        hipLaunchKernelGGL(k_bq_pass<Pol>, dim3((unsigned)pl.bq_pass_blocks), dim3(256),
-                           (size_t)P.n3 * (sizeof(double) + sizeof(float)), stream, P, Q);
+        // the mixed pass kernel is held to 4 waves per SIMD (128 VGPRs), the
+        // FP64 one runs at its natural register count
+        constexpr int wps = sizeof(typename Pol::real) == 4 ? 4 : 1;
+        hipLaunchKernelGGL((k_bq_pass<Pol, wps>), dim3((unsigned)pl.bq_pass_blocks), dim3(256),
+                           (size_t)P.n3 * (sizeof(kp::bq::tile_t<Pol>) + sizeof(float)),
+                           stream, P, Q);
         if (Q.pr) {
             hipLaunchKernelGGL(k_bq_peel<Pol>, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);

@@ -12,7 +12,7 @@ the same histories as the other schedules:
     back to the queue every few steps; J and images equal to the order of
     FP64 sums;
   - FP32: end states identical to the persistent mixed-precision kernel's,
-    J to the FP32 rounding of its register accumulator;
+    J to the FP32 rounding of the queue's per-task LDS tile;
   - a captured graph replays the same sequence.
 """
 import numpy as np
@@ -157,7 +157,8 @@ def test_f32_equals_persistent_mixed(passes, yld):
     _same_states(b["end_state"], a["end_state"])
     assert np.array_equal(a["counters"], b["counters"])
     _close(b["image"], a["image"], 1e-9)
-    _close(b["J"], a["J"], 2e-6)
+    # the queue sums each task's J in an FP32 LDS tile before the FP64 atomic
+    _close(b["J"], a["J"], 3e-5)
 
 
 def test_graph_replays_the_queue():
@@ -177,7 +178,9 @@ def test_graph_replays_the_queue():
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out["counters"].cpu().numpy(),
                                       (k + 1) * eager["counters"].cpu().numpy())
-        _close(out["J"].cpu().numpy(), (k + 1) * eager["J"].cpu().numpy(), 1e-11)
+        # J: each task's FP32 LDS tile sums its packets in LDS-atomic order,
+        # which differs between launches (FP32 rounding, ~1e-7)
+        _close(out["J"].cpu().numpy(), (k + 1) * eager["J"].cpu().numpy(), 1e-5)
         _close(out["image"].cpu().numpy(), (k + 1) * eager["image"].cpu().numpy(), 1e-11)
 
 
