@@ -28,11 +28,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 # HBM traffic per algorithmic byte of the mixed kernel on the config-2
-# workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one dispatch of
-# 5e6 packets: 28.05 GB fetched, after scaling by the 2.000 a 512 MiB copy
-# calls for, plus 24.57 GB written = 52.62 GB, against 30.35 GB algorithmic;
-# profiles/k_persist_pol32_config2_amr3.md).  Both scale with the packets.
-TRAFFIC_PER_ALG_BYTE = {"config2_amr3_r2_lognormal_hg": 52.65 / 30.34}
+# workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the timed
+# dispatch of 1e8 packets: 563.08 GB fetched (after scaling by the 2.000 a
+# 512 MiB copy calls for) plus 490.39 GB written = 1053.47 GB, against
+# 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md; a
+# 5e6-packet dispatch gave 1.735).
+TRAFFIC_PER_ALG_BYTE = {"config2_amr3_r2_lognormal_hg": 1053.47 / 606.92}
 T_START = time.perf_counter()
 
 
@@ -999,7 +1000,8 @@ def main():
                      "traffic": (TRAFFIC_PER_ALG_BYTE[w.name] * b_alg
                                  if w.name in TRAFFIC_PER_ALG_BYTE else None),
                      "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic "
-                                        "byte, profiles/k_persist_pol32_config2_amr3.md"
+                                        "byte of the 1e8-packet dispatch, "
+                                        "profiles/k_persist_pol32_config2_amr3.md"
                                         if w.name in TRAFFIC_PER_ALG_BYTE else None),
                      "alg_bytes_per_step": b_alg,
                      "kernel_ms_per_step": 1e3 * kern32 / args.steps},

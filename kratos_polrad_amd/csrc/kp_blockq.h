@@ -10,11 +10,13 @@
 //     cell, counts), keyed by (band, block);
 //   - a pass sorts the records by key (stable LSD radix sort) and gives each
 //     workgroup a run of sorted positions; per block of that run the
-//     workgroup stages the block's kappa_rho (n_cell^3 floats) and an FP64 J
-//     tile in LDS, marches each of its packets until the packet leaves the
-//     block or ends (interactions and peel-offs included; peel-off rays read
-//     kappa_rho from HBM), deposits J into the LDS tile, and flushes the tile
-//     once with one FP64 atomic per touched cell;
+//     workgroup stages the block's kappa_rho (n_cell^3 floats) and a J tile
+//     (FP32 in the mixed path, FP64 in the FP64 path) in LDS, marches each of
+//     its packets until the packet leaves the block or ends, deposits J into
+//     the LDS tile, and flushes the tile once with one FP64 atomic per
+//     touched cell;
+//   - an interaction queues what its peel-offs are made of; the peel kernel
+//     computes and marches them after the pass (kappa_rho from HBM);
 //   - a packet that crosses into another block is written to the next pass's
 //     records at a slot claimed once per wave (ballot / prefix rank), with
 //     its new key; one that ends is counted and its end state written.
@@ -55,17 +57,22 @@ template <> struct __align__(16) rec_t<float> {
 };
 static_assert(sizeof(rec_t<double>) == 144 && sizeof(rec_t<float>) == 96, "record sizes");
 
-// A peel-off ray of a queued packet, marched by the peel kernel after the
-// pass: its start (the interaction point: position as in rec_t, block,
-// cell), observer, packet slot, and the Stokes vector it carries before
-// attenuation.
+// The peel-offs of one interaction of a queued packet, computed and marched
+// by the peel kernel after the pass: the interaction point (position as in
+// rec_t, block, cell), the packet slot, and what the peel-off Stokes vectors
+// toward the observers are made of -- the incoming direction and Stokes
+// frame, the Stokes vector after the albedo weighting, and the cell's g.
+// 64 B in FP32, 128 B in FP64.
 template <class R> struct __align__(16) ray_rec_t {
     R x[3];
     int32_t ib;
-    uint32_t cell_o;                 // i0 | i1 << 8 | i2 << 16 | observer << 24
+    uint32_t cell;                   // i0 | i1 << 8 | i2 << 16
     uint32_t slot;
-    R s[3];
+    R d[3], e1[3], s[3];
+    R g;
 };
+static_assert(sizeof(ray_rec_t<float>) == 64 && sizeof(ray_rec_t<double>) == 128,
+              "peel-off record sizes");
 
 // Launch parameters of the queue kernels (besides kparams).
 struct qparams {
@@ -181,31 +188,58 @@ __device__ __forceinline__ uint64_t claim(unsigned long long *count)
     return base + rank;
 }
 
-// Queue the peel-off ray toward observer o (Stokes vector S before
-// attenuation) for the peel kernel; false when the queue is full.  Called
-// by the interacting lanes together.
+// Queue the peel-offs of the lane's interaction (cell g gg; L holds the
+// incoming direction and frame and the albedo-weighted Stokes vector) for
+// the peel kernel; false when the queue is full.  Called by the interacting
+// lanes together.  Slots come from the wave's chunk of the queue (one
+// counter atomic per chunk of kQChunk records; wave_peel_open first,
+// wave_peel_close when the wave is done).
 template <class Pol>
-__device__ __forceinline__ bool push_ray(const qparams &Q, const qlane<Pol> &L, int o,
-                                         typename Pol::real S0, typename Pol::real S1,
-                                         typename Pol::real S2)
+__device__ __forceinline__ bool push_peel(const qparams &Q, const qlane<Pol> &L,
+                                          typename Pol::real gg)
 {
     using real = typename Pol::real;
-    const uint64_t k = claim(Q.pr_count);
+    const uint64_t k = claim_q(Q.pr_count, kp_pq_base, kp_pq_end);
     if (k >= Q.pr_cap)
         return false;
     ray_rec_t<real> r;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         r.x[a] = L.r.x[a];
+        r.d[a] = L.r.d[a];
+        r.e1[a] = L.e1[a];
+    }
     r.ib = L.r.ib;
-    r.cell_o = (uint32_t)L.r.i[0] | ((uint32_t)L.r.i[1] << 8) | ((uint32_t)L.r.i[2] << 16) |
-               ((uint32_t)o << 24);
+    r.cell = (uint32_t)L.r.i[0] | ((uint32_t)L.r.i[1] << 8) | ((uint32_t)L.r.i[2] << 16);
     r.slot = (uint32_t)L.slot;
-    r.s[0] = S0;
-    r.s[1] = S1;
-    r.s[2] = S2;
+    r.s[0] = L.I;
+    r.s[1] = L.Q;
+    r.s[2] = L.U;
+    r.g = gg;
     static_cast<ray_rec_t<real> *>(Q.pr)[k] = r;
     return true;
+}
+
+// The calling wave's peel-off chunk: none yet.
+__device__ __forceinline__ void wave_peel_open()
+{
+    if ((threadIdx.x & 63) == 0) {
+        kp_pq_base[threadIdx.x >> 6] = 0;
+        kp_pq_end[threadIdx.x >> 6] = 0;
+    }
+}
+
+// Marks the unused rest of the calling wave's peel-off chunk (ib = -1; the
+// peel kernel skips such records).  Called by the whole wave when it is
+// done.
+template <class R> __device__ __forceinline__ void wave_peel_close(const qparams &Q)
+{
+    if (!Q.pr)
+        return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t e = kp_pq_end[w] < Q.pr_cap ? kp_pq_end[w] : Q.pr_cap;
+    for (uint64_t k = kp_pq_base[w] + lane; k < e; k += 64)
+        static_cast<ray_rec_t<R> *>(Q.pr)[k].ib = -1;
 }
 
 template <class Pol>
@@ -225,15 +259,15 @@ __device__ bool interact(const kparams &P, const qparams &Qp, qlane<Pol> &L, int
         L.U = L.U * alb;
         double e2[3];
         f64::cross3(L.r.d, L.e1, e2);
-        for (int o = 0; o < P.n_incl; ++o) {
-            double Ip, Qi, Ui;
-            f64::peel_stokes(P.model, gg, P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, P.no[o],
-                             P.ev[o], Ip, Qi, Ui);
-            if (Qp.pr && push_ray<Pol>(Qp, L, o, Ip, Qi, Ui))
-                continue;
-            const double tau = f64::tau_to_edge(P, L.r, P.no[o], L.ks, np);
-            const double att = kp_exp(-tau) * KP_INV_4PI;
-            f64::deposit(P, L.band, o, L.r.x, Ip * att, Qi * att, Ui * att);
+        if (P.n_incl > 0 && !(Qp.pr && push_peel<Pol>(Qp, L, gg))) {
+            for (int o = 0; o < P.n_incl; ++o) {
+                double Ip, Qi, Ui;
+                f64::peel_stokes(P.model, gg, P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, P.no[o],
+                                 P.ev[o], Ip, Qi, Ui);
+                const double tau = f64::tau_to_edge(P, L.r, P.no[o], L.ks, np);
+                const double att = kp_exp(-tau) * KP_INV_4PI;
+                f64::deposit(P, L.band, o, L.r.x, Ip * att, Qi * att, Ui * att);
+            }
         }
         const double mu = (P.model == KP_THOMSON) ? kp_thomson_mu(kp_u01(u[0]))
                                                   : kp_hg_mu(gg, kp_u01(u[0]));
@@ -257,7 +291,7 @@ __device__ bool interact(const kparams &P, const qparams &Qp, qlane<Pol> &L, int
         L.U *= alb;
         float e2[3];
         f32::cross3(L.r.d, L.e1, e2);
-        if (P.n_incl > 0) {
+        if (P.n_incl > 0 && !(Qp.pr && push_peel<Pol>(Qp, L, gg))) {
             float gx[3];
             f32::global_pos(P, L.r, gx);
             for (int o = 0; o < P.n_incl; ++o) {
@@ -267,8 +301,6 @@ __device__ bool interact(const kparams &P, const qparams &Qp, qlane<Pol> &L, int
                 float Ip, Qi, Ui;
                 f32::peel_stokes(P.model, gg, (float)P.pmax, L.r.d, L.e1, e2, L.I, L.Q, L.U, no,
                                  ev, Ip, Qi, Ui);
-                if (Qp.pr && push_ray<Pol>(Qp, L, o, Ip, Qi, Ui))
-                    continue;
                 const double tau = f32::tau_to_edge(P, L.r, no, L.ks, np);
                 const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
                 f32::deposit(P, L.band, o, gx, Ip * att, Qi * att, Ui * att);
@@ -435,9 +467,11 @@ __device__ __forceinline__ void end_packet(const kparams &P, const qlane<Pol> &L
     }
 }
 
-// A queued peel-off ray marched to the domain edge (kappa_rho from HBM), its
-// attenuated Stokes vector deposited at the pixel of its start.  Returns the
-// ray's crossings.
+// The queued peel-offs of one interaction: per observer the peel-off Stokes
+// vector (the interaction's own arithmetic, on the recorded direction,
+// frame, Stokes vector and g), the ray marched to the domain edge
+// (kappa_rho from HBM), the attenuated vector deposited at the pixel of the
+// interaction point.  Returns the rays' crossings.
 template <class Pol>
 __device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<typename Pol::real> &q)
 {
@@ -449,24 +483,37 @@ __device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<t
         t.d[a] = 0;
     }
     t.ib = q.ib;
-    unpack_cell<Pol>(t, q.cell_o);
-    const int o = (int)(q.cell_o >> 24);
+    unpack_cell<Pol>(t, q.cell);
     const int band = band_of(P, q.slot);
     const real ks = (real)P.bands[band].kappa_scale;
+    real e2[3];
     uint64_t np = 0;
     if constexpr (sizeof(real) == 8) {
+        f64::cross3(q.d, q.e1, e2);
         f64::gl_load(P, t);
-        const double tau = f64::tau_to_edge(P, t, P.no[o], ks, np);
-        const double att = kp_exp(-tau) * KP_INV_4PI;
-        f64::deposit(P, band, o, t.x, q.s[0] * att, q.s[1] * att, q.s[2] * att);
+        for (int o = 0; o < P.n_incl; ++o) {
+            double Ip, Qi, Ui;
+            f64::peel_stokes(P.model, q.g, P.pmax, q.d, q.e1, e2, q.s[0], q.s[1], q.s[2],
+                             P.no[o], P.ev[o], Ip, Qi, Ui);
+            const double tau = f64::tau_to_edge(P, t, P.no[o], ks, np);
+            const double att = kp_exp(-tau) * KP_INV_4PI;
+            f64::deposit(P, band, o, t.x, Ip * att, Qi * att, Ui * att);
+        }
     } else {
-        float no[3];
-        load3(P.no[o], no);
+        f32::cross3(q.d, q.e1, e2);
         float gx[3];
         f32::global_pos(P, t, gx);
-        const double tau = f32::tau_to_edge(P, t, no, ks, np);
-        const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
-        f32::deposit(P, band, o, gx, q.s[0] * att, q.s[1] * att, q.s[2] * att);
+        for (int o = 0; o < P.n_incl; ++o) {
+            float no[3], ev[3];
+            load3(P.no[o], no);
+            load3(P.ev[o], ev);
+            float Ip, Qi, Ui;
+            f32::peel_stokes(P.model, q.g, (float)P.pmax, q.d, q.e1, e2, q.s[0], q.s[1],
+                             q.s[2], no, ev, Ip, Qi, Ui);
+            const double tau = f32::tau_to_edge(P, t, no, ks, np);
+            const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
+            f32::deposit(P, band, o, gx, Ip * att, Qi * att, Ui * att);
+        }
     }
     if (P.end_state)
         atomicAdd(&P.end_state[(uint64_t)q.slot * KP_ES_STRIDE + 14], (double)np);

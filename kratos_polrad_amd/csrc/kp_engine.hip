@@ -532,6 +532,7 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     kp::lane_counters C;
     const uint64_t n = *Q.n_in;
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
+    kp::bq::wave_peel_open();
     for (uint64_t t = blockIdx.x; t * Q.chunk < n; t += gridDim.x) {
         uint64_t p = t * Q.chunk;
         const uint64_t p1 = (p + Q.chunk < n) ? p + Q.chunk : n;
@@ -578,11 +579,13 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
             p = e;
         }
     }
+    kp::bq::wave_peel_close<real>(Q);
     flush_counters(P, C);
 }
 
-// The peel-off rays queued by a pass: one ray per lane, in queue order (the
-// rays a wave queued start in one block and run parallel to one observer).
+// The peel-offs queued by a pass: one interaction per lane, in queue order,
+// its rays toward the observers one after the other (the interactions a
+// wave queued lie in one block, so the lanes' rays run parallel).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_bq_peel(kp::kparams P, kp::bq::qparams Q)
 {
@@ -592,7 +595,8 @@ __global__ void __launch_bounds__(256) k_bq_peel(kp::kparams P, kp::bq::qparams 
     const auto *rays = static_cast<const kp::bq::ray_rec_t<real> *>(Q.pr);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
-        C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
+        if (rays[k].ib >= 0) // not the unused rest of a wave's chunk
+            C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
     flush_counters(P, C);
 }
 
@@ -605,12 +609,14 @@ __global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparam
     const uint64_t n = *Q.n_in;
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    kp::bq::wave_peel_open();
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         kp::bq::qlane<Pol> L;
         kp::bq::load<Pol>(P, in[k], L);
         (void)kp::bq::march<Pol, false>(P, Q, L, nullptr, nullptr);
         kp::bq::end_packet<Pol>(P, L, C);
     }
+    kp::bq::wave_peel_close<real>(Q);
     flush_counters(P, C);
 }
 
@@ -1206,7 +1212,7 @@ struct fwd_plan {
     int bq_key_bits = 1, bq_passes = 32, n_blocks = 0;
     uint32_t bq_chunk = 256, bq_yield = 128;
     uint64_t bq_pass_blocks = 0, bq_finish_blocks = 0;
-    uint64_t bq_ray_cap = 0; // peel-off rays a pass can queue (0: marched in place)
+    uint64_t bq_ray_cap = 0; // peel-off records a pass can queue (0: marched in place)
 };
 
 // Block-queue buffers of a plan, grown to n records of rec_bytes each.
@@ -1336,13 +1342,13 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
             const size_t rb = run->precision == KP_F64 ? sizeof(kp::bq::rec_t<double>)
                                                        : sizeof(kp::bq::rec_t<float>);
             ensure_bq(fb, pl.bq_n_slots, rb, pl.bq_n_keys, run->n_bands);
-            // peel-off rays of a pass: twice one ray per packet and observer,
-            // at least 2^20 and at most 2^27 rays or 8 GB; rays beyond it are
-            // marched by the lane that made them
+            // peel-off records of a pass (one per interaction, all observers):
+            // two per packet, at least 2^20 and at most 2^27 records or 8 GB;
+            // interactions beyond it march their peel-offs in place
             if (run->n_incl > 0) {
                 const size_t rr = run->precision == KP_F64 ? sizeof(kp::bq::ray_rec_t<double>)
                                                            : sizeof(kp::bq::ray_rec_t<float>);
-                uint64_t cap = 2 * pl.bq_n_slots * (uint64_t)run->n_incl;
+                uint64_t cap = 2 * pl.bq_n_slots;
                 cap = std::max<uint64_t>(cap, 1ull << 20);
                 cap = std::min<uint64_t>(cap, std::min<uint64_t>(1ull << 27, (8ull << 30) / rr));
                 pl.bq_ray_cap = cap;
