@@ -186,17 +186,22 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *
     r.d[2] = dir[2];
     set_scales(P, r);
     double tau = 0.0;
+    // kappa_rho of the next cell is loaded one step ahead: the crossing is
+    // taken before the current cell's optical depth is added (the ray is a
+    // copy, so the crossing after the last cell is simply dropped)
+    float kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
     for (;;) {
         float l;
         const int ap = get_dl(r, l);
         if (ap < 0)
             break;
-        const float kap = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)] * ks;
+        const float kap = kc * ks;
+        const int st = cross_face(P, r, ap, l);
+        if (st == ST_IN)
+            kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
         tau += (double)(kap * l);
         ncross += 1;
-        if (tau > kTauCut)
-            break;
-        if (cross_face(P, r, ap, l) != ST_IN)
+        if (tau > kTauCut || st != ST_IN)
             break;
     }
     return tau;

@@ -205,22 +205,27 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const double 
     r.d[1] = dir[1];
     r.d[2] = dir[2];
     double tau = 0.0;
+    // kappa_rho of the next cell is loaded one step ahead: the crossing is
+    // taken before the current cell's optical depth is added (the ray is a
+    // copy, so the crossing after the last cell is simply dropped)
+    float kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
     for (;;) {
         double l;
         const int ap = get_dl(r, l);
         if (ap < 0)
             break;
-        const double kap = (double)P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)] * ks;
-        tau = tau + kap * l;
-        ncross += 1;
-        if (tau > kTauCut)
-            break;
+        const double kap = (double)kc * ks;
         bool reload;
         const int st = cross_face(P, r, ap, l, reload);
-        if (st != ST_IN)
+        if (st == ST_IN) {
+            if (reload)
+                gl_load(P, r);
+            kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
+        }
+        tau = tau + kap * l;
+        ncross += 1;
+        if (tau > kTauCut || st != ST_IN)
             break;
-        if (reload)
-            gl_load(P, r);
     }
     return tau;
 }
