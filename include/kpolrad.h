@@ -242,15 +242,17 @@ typedef struct kp_tallies {
  *   boundary/comm.h:118-127, with blocks as destinations).  Bands from
  *   queue_from_band on run as passes: the packets' records (144 B FP64,
  *   96 B FP32) are sorted by (band, block); a workgroup takes queue_chunk
- *   sorted packets, stages each of their blocks' kappa_rho and an FP64 J tile
- *   in LDS, and marches every packet until it leaves the block or ends
- *   (interactions and peel-offs included), then adds the tile to J with one
- *   atomic per touched cell; packets that cross into another block are
- *   compacted into the next pass's records by wave ballot and prefix rank.
- *   A packet that takes queue_yield steps in one pass without leaving its
- *   block goes back to the queue.  Peel-off rays are queued and marched by
- *   a kernel of their own after each pass, one ray per lane.  After
- *   queue_passes passes the packets still alive finish one thread each.  Bands before queue_from_band run on the persistent kernel in the
+ *   sorted packets, stages each of their blocks' kappa_rho and a J tile
+ *   (FP32 in the mixed path, FP64 in the FP64 path) in LDS, and marches
+ *   every packet until it leaves the block or ends, then adds the tile to J
+ *   with one FP64 atomic per touched cell; packets that cross into another
+ *   block are compacted into the next pass's records by wave ballot and
+ *   prefix rank.  A packet that takes queue_yield steps in one pass without
+ *   leaving its block goes back to the queue.  Each interaction queues what
+ *   its peel-offs are made of (64 B FP32, 128 B FP64); a kernel of their own
+ *   computes and marches them after each pass, one interaction per lane.
+ *   After queue_passes passes the packets still alive finish one thread
+ *   each.  Bands before queue_from_band run on the persistent kernel in the
  *   same launch.  Same histories; needs n_cell^3 <= 4096 and fewer than 2^31
  *   queued packets per launch.  Record memory: 2 x record size + 16 B per
  *   queued packet, kept by the engine. */
@@ -319,8 +321,8 @@ typedef struct kp_schedule {
     int32_t adjoint_chunk;
     /* KP_SCHED_BLOCKQ: first band of the block queue (0 = every band; the
      * bands before it run on the persistent kernel), passes before the
-     * packets still alive finish one thread each (0 = 16), sorted packets
-     * per workgroup task (0 = 256), and the march steps plus interactions a
+     * packets still alive finish one thread each (0 = 20), sorted packets
+     * per workgroup task (0 = 1024), and the march steps plus interactions a
      * packet may take in one pass before it goes back to the queue in its
      * block (0 = 128), which bounds the slowest lane of a pass. */
     int32_t queue_from_band;

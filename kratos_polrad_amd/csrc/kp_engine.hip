@@ -1209,8 +1209,8 @@ struct fwd_plan {
     int bq_first_band = 0;
     uint64_t bq_first_slot = 0, bq_n_slots = 0;
     uint32_t bq_n_keys = 0;
-    int bq_key_bits = 1, bq_passes = 32, n_blocks = 0;
-    uint32_t bq_chunk = 256, bq_yield = 128;
+    int bq_key_bits = 1, bq_passes = 20, n_blocks = 0;
+    uint32_t bq_chunk = 1024, bq_yield = 128;
     uint64_t bq_pass_blocks = 0, bq_finish_blocks = 0;
     uint64_t bq_ray_cap = 0; // peel-off records a pass can queue (0: marched in place)
 };
@@ -1322,8 +1322,8 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         for (int b = 0; b < pl.bq_first_band; ++b)
             pl.bq_first_slot += run->bands[b].n_packets;
         pl.bq_n_slots = total - pl.bq_first_slot;
-        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 16;
-        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 256u;
+        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 20;
+        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 1024u;
         pl.bq_yield = S.queue_yield > 0 ? (uint32_t)S.queue_yield : 128u;
         pl.bq_pass_blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
         pl.bq_finish_blocks = (uint64_t)e->n_cu * 4;
@@ -2265,8 +2265,8 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 if (queue_best < best_ms) {
                     best_ms = queue_best;
                     cur = queue_pick;
-                    try_knob(&kp_schedule::queue_chunk, {128, 512});
-                    try_knob(&kp_schedule::queue_passes, {8, 24, 32});
+                    try_knob(&kp_schedule::queue_chunk, {256, 2048});
+                    try_knob(&kp_schedule::queue_passes, {12, 16, 28});
                     try_knob(&kp_schedule::queue_yield, {64, 256});
                     try_knob(&kp_schedule::blocks_per_cu, {4, 8, 16});
                 }
