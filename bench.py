@@ -27,13 +27,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
-# HBM traffic per algorithmic byte of the mixed kernel on the config-2
-# workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the timed
-# dispatch of 1e8 packets: 563.08 GB fetched (after scaling by the 2.000 a
-# 512 MiB copy calls for) plus 490.39 GB written = 1053.47 GB, against
-# 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md; a
-# 5e6-packet dispatch gave 1.735).
-TRAFFIC_PER_ALG_BYTE = {"config2_amr3_r2_lognormal_hg": 1053.47 / 606.92}
+# HBM traffic per algorithmic byte of the mixed path on the config-2
+# workload, per schedule kind, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on the timed step of 1e8 packets (FETCH_SIZE scaled by the
+# 2.000 a 512 MiB copy calls for):
+#   persistent kernel: 563.08 GB fetched + 490.39 GB written = 1053.47 GB
+#   against 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md;
+#   a 5e6-packet dispatch gave 1.735);
+#   block queue, all dispatches of the step: 282.17 + 273.84 = 556.02 GB
+#   (profiles/block_queue_config2_amr3.md): below the algorithmic bytes,
+#   since the J tile takes the FP64 read-modify-writes of most crossings.
+TRAFFIC_PER_ALG_BYTE = {("config2_amr3_r2_lognormal_hg", 1): 1053.47 / 606.92,
+                        ("config2_amr3_r2_lognormal_hg", 2): 556.02 / 606.92}
+TRAFFIC_SOURCE = {("config2_amr3_r2_lognormal_hg", 1): (
+    "rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic byte of the 1e8-packet "
+    "k_persist dispatch, profiles/k_persist_pol32_config2_amr3.md"),
+    ("config2_amr3_r2_lognormal_hg", 2): (
+    "rocprofv3 FETCH_SIZE + WRITE_SIZE summed over the dispatches of one "
+    "1e8-packet block-queue step, per algorithmic byte, "
+    "profiles/block_queue_config2_amr3.md")}
+# what one timed step launches, per schedule kind (the roofline's "kernel":
+# achieved = algorithmic bytes of the step / HIP-event time of the step)
+KERNELS_OF_KIND = {
+    0: "k_forward_f32 (one thread per packet)",
+    1: "k_persist (one persistent launch) and the replica folds",
+    2: "block queue: emission peel-offs, then per pass a key sort, k_bq_pass and "
+       "k_bq_peel; k_bq_finish and the folds",
+}
 T_START = time.perf_counter()
 
 
@@ -79,6 +99,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--skip-f64", action="store_true")
+    p.add_argument("--main-kind", default="tuned", choices=["tuned", "persistent", "block_queue"],
+                   help="schedule kind of the main workload: the autotuner's pick screened at "
+                        "full size, or that kind with the tuned knobs")
     p.add_argument("--skip-adjoint", action="store_true")
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
@@ -916,6 +939,7 @@ def main():
     n_incl = len(w.run.incl_deg)
 
     eb_check = {}
+    main_kind = args.main_kind
 
     def timed(prec, steps, warmup, offset0=0):
         run = RunSpec(**{**w.run.__dict__, "precision": prec})
@@ -924,16 +948,28 @@ def main():
         # the image replicas inconsistently (up to 7 % apart at 1e8 packets)
         eng.autotune(w.source, run, sample_packets=min(20_000_000, n))
         out = eng.alloc_outputs(run)
-        # the sample still favours a small event batch that is slower at full
-        # size (833 against 795 ms per 1e8 mixed packets): the tuned batch
-        # and the other of 8 / 16 are timed again at full size, off the timed
-        # packet ids, and the faster runs
+        # the sample decides on 2e7 packets, and some choices pay off
+        # differently at full size: for the persistent kernel the sample
+        # favours a small event batch that is slower at 1e8 packets (833
+        # against 795 ms per 1e8 mixed packets), and the block queue's gain
+        # grows with the packets per block.  So the tuned schedule and its
+        # alternative (the other of event batch 8 / 16, or the persistent
+        # kernel for the queue) are timed again at full size, off the timed
+        # packet ids, and the faster runs.  --main-kind fixes the kind.
         tuned = eng.get_schedule()
+        if main_kind == "persistent":
+            tuned = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
+        elif main_kind == "block_queue":
+            tuned = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0}
         eb = tuned["event_batch"]
-        cands = [eb, 8 if eb == 16 else 16]
+        cands = [tuned, {**tuned, "event_batch": 8 if eb == 16 else 16}]
+        if tuned["kind"] == _abi.KP_SCHED_BLOCKQ:
+            cands[1] = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
+        if main_kind == "block_queue":
+            cands = cands[:1]
         check = []
         for c in cands:
-            eng.set_schedule(**{**tuned, "event_batch": c})
+            eng.set_schedule(**c)
             run.packet_offset = 1 << 51
             eng.forward(w.source, run, out)
             torch.cuda.synchronize()
@@ -941,9 +977,10 @@ def main():
             eng.forward(w.source, run, out)
             torch.cuda.synchronize()
             check.append(time.perf_counter() - t)
-        eng.set_schedule(**{**tuned, "event_batch": cands[int(np.argmin(check))]})
+        eng.set_schedule(**cands[int(np.argmin(check))])
         eb_check["mixed" if prec == _abi.KP_F32 else "fp64"] = {
-            "event_batch": cands, "ms": [1e3 * c for c in check]}
+            "candidates": [{"kind": c["kind"], "event_batch": c["event_batch"]} for c in cands],
+            "ms": [1e3 * c for c in check]}
         for k in range(warmup):
             run.packet_offset = (1 << 50) + k * n
             eng.forward(w.source, run, out)
@@ -974,6 +1011,7 @@ def main():
     value = cross32 / wall32
     b_alg = algorithmic_bytes(cnt32, n_incl) / args.steps
     achieved = b_alg / (kern32 / args.steps)
+    tkey = (w.name, sched32["kind"])
 
     res = {
         "metric": "cell-crossings/s at fixed Stokes-image error vs FP64",
@@ -994,15 +1032,13 @@ def main():
                    "schedule": sched32},
         "crossings_per_step": {"main": cnt32[0] / args.steps, "peel": cnt32[1] / args.steps,
                                "events": cnt32[2] / args.steps},
-        "event_batch_check": eb_check,
+        "full_size_check": eb_check,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (TRAFFIC_PER_ALG_BYTE[w.name] * b_alg
-                                 if w.name in TRAFFIC_PER_ALG_BYTE else None),
-                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic "
-                                        "byte of the 1e8-packet dispatch, "
-                                        "profiles/k_persist_pol32_config2_amr3.md"
-                                        if w.name in TRAFFIC_PER_ALG_BYTE else None),
+                     "kernels": KERNELS_OF_KIND[sched32["kind"]],
+                     "traffic": (TRAFFIC_PER_ALG_BYTE[tkey] * b_alg
+                                 if tkey in TRAFFIC_PER_ALG_BYTE else None),
+                     "traffic_source": TRAFFIC_SOURCE.get(tkey),
                      "alg_bytes_per_step": b_alg,
                      "kernel_ms_per_step": 1e3 * kern32 / args.steps},
     }
