@@ -344,11 +344,14 @@ kp_status kp_get_schedule(const kp_engine *eng, kp_schedule *sched);
 kp_status kp_forward(kp_engine *eng, const kp_source *src, const kp_run *run,
                      kp_tallies *out, void *stream);
 
-/* Autotuner: times candidate schedules (action-phase batch, register budget,
- * replica counts, blocks per CU, block size; coordinate descent from the
- * current schedule) on a sample of about sample_packets packets of `run`
- * with HIP events on `stream`, installs the fastest and returns it.  Results
- * are cached per workload shape (precision, bands, observers, image size,
+/* Autotuner: times candidate schedules (kind and packet order, action-phase
+ * batch, register budget, replica counts, blocks per CU, block size;
+ * coordinate descent from the current schedule; then the block queue for
+ * every band and for the longest tail of bands whose extinction scale is at
+ * most 1, 0.5, 0.25 or 0.1, and, if one of them wins, its task size, pass
+ * count, yield and blocks per CU) on a sample of about sample_packets
+ * packets of `run` with HIP events on `stream`, installs the fastest and
+ * returns it.  Results are cached per workload shape (precision, bands, observers, image size,
  * scattering model, mesh size, packet-count octave); force != 0 re-times.
  * Blocking call.  No reference counterpart (the reference takes n_th from
  * the input file, driver_base.h:30). */
