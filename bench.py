@@ -565,15 +565,19 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
     res = {"workload": w.name, "blocks": w.mesh.n_blocks, "cells": w.mesh.n_cells,
            "levels": int(w.mesh.level.max()) + 1, "packets": packets, "chunk": chunk,
            "n_incl": len(w.run.incl_deg), "image_n": w.run.image_n, "setup_s": t_setup}
-    # FP64 and mixed on the same first chunk, each precision with its tuned
-    # schedule in the faster packet order at this size (see batched)
+    # FP64 and mixed on the same first chunk, each precision with the fastest
+    # at this size of its tuned schedule in both packet orders and the block
+    # queue for every band (see batched)
     runs, scheds = {}, {}
     for prec in (_abi.KP_F64, _abi.KP_F32):
         run = RunSpec(**{**w.run.__dict__, "precision": prec})
         tuned = eng.autotune(w.source, run, sample_packets=4_000_000)
         best = None
-        for order in (0, 1):
-            sc = {**tuned, "coherent": order}
+        cands = [{**tuned, "coherent": 0}, {**tuned, "coherent": 1}]
+        if tuned["kind"] != _abi.KP_SCHED_BLOCKQ:
+            cands.append({**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0,
+                          "coherent": 0})
+        for sc in cands:
             eng.set_schedule(**sc)
             o = eng.alloc_outputs(run)
             torch.cuda.synchronize()
@@ -586,7 +590,8 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
                 best = (dt, (c[0] + c[1]) / dt, o["image"].cpu().numpy())
                 scheds[prec] = sc
             del o
-            progress(f"large: one chunk, precision {prec}, order {order}, {dt:.1f} s")
+            progress(f"large: one chunk, precision {prec}, kind {sc['kind']}, "
+                     f"order {sc['coherent']}, {dt:.1f} s")
         runs[prec] = best
     res["fp64_one_chunk"] = {"seconds": runs[_abi.KP_F64][0], "value": runs[_abi.KP_F64][1],
                              "schedule": scheds[_abi.KP_F64]}

@@ -13,7 +13,8 @@ the same histories as the other schedules:
     FP64 sums;
   - FP32: end states identical to the persistent mixed-precision kernel's,
     J to the FP32 rounding of the queue's per-task LDS tile;
-  - a captured graph replays the same sequence.
+  - a captured graph replays the same sequence, and a graph of several
+    chunks equals the chunks launched one by one.
 """
 import numpy as np
 import pytest
@@ -182,6 +183,33 @@ def test_graph_replays_the_queue():
         # which differs between launches (FP32 rounding, ~1e-7)
         _close(out["J"].cpu().numpy(), (k + 1) * eager["J"].cpu().numpy(), 1e-5)
         _close(out["image"].cpu().numpy(), (k + 1) * eager["image"].cpu().numpy(), 1e-11)
+
+
+@pytest.mark.parametrize("precision", [_abi.KP_F64, _abi.KP_F32], ids=["f64", "f32"])
+def test_graph_of_chunks_equals_eager_chunks(precision):
+    """A graph of three chunks (packet ids offset per chunk, as the large
+    workload streams them) against the same chunks launched one by one."""
+    import torch
+
+    w = _amr_workload(3000, precision=precision)
+    print(f"blockq graph of 3 chunks, precision {precision}", flush=True)
+    eng = Engine(w.mesh)
+    eng.set_schedule(**_bq(passes=6, chunk=512))
+    eng.set_media(w.kappa, w.albedo, w.g)
+    stride = w.run.total_packets
+    ref = eng.alloc_outputs(w.run)
+    for k in range(3):
+        eng.forward(w.source, RunSpec(**{**w.run.__dict__, "packet_offset": k * stride}), ref)
+    out = eng.alloc_outputs(w.run)
+    g = eng.graph(w.source, w.run, out, n_chunks=3)
+    g.launch()
+    torch.cuda.synchronize()
+    got = {k: v.cpu().numpy() for k, v in out.items()}
+    exp = {k: v.cpu().numpy() for k, v in ref.items()}
+    assert np.array_equal(got["counters"], exp["counters"])
+    # J: FP64 sums in another order; FP32 also the tile's LDS-atomic order
+    _close(got["J"], exp["J"], 1e-12 if precision == _abi.KP_F64 else 1e-5)
+    _close(got["image"], exp["image"], 1e-11)
 
 
 def test_config3_graph_queue_bands_fp64_equal_oracle(oracle_lib):
