@@ -586,8 +586,8 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
 // The peel-offs queued by a pass: one interaction per lane, in queue order,
 // its rays toward the observers one after the other (the interactions a
 // wave queued lie in one block, so the lanes' rays run parallel).
-template <class Pol>
-__global__ void __launch_bounds__(256) k_bq_peel(kp::kparams P, kp::bq::qparams Q)
+template <class Pol, int WPS>
+__global__ void __launch_bounds__(256, WPS) k_bq_peel(kp::kparams P, kp::bq::qparams Q)
 {
     using real = typename Pol::real;
     kp::lane_counters C;
@@ -1398,6 +1398,11 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
     return pl;
 }
 
+// Waves per SIMD the peel kernel is compiled for: the mixed kernel's march is
+// bound by kappa_rho load latency, so it keeps 8 waves per SIMD resident (64
+// VGPRs); the FP64 one runs at its natural register count.
+template <class Pol> constexpr int kPeelWaves = sizeof(typename Pol::real) == 4 ? 8 : 1;
+
 // The block-queue part of a forward run (kernels only): emission peel-offs
 // and records, queue_passes passes (sort by key, segment table, pass), then
 // the packets still alive to their end.
@@ -1457,7 +1462,8 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
                            (size_t)P.n3 * (sizeof(kp::bq::tile_t<Pol>) + sizeof(float)),
                            stream, P, Q);
         if (Q.pr) {
-            hipLaunchKernelGGL(k_bq_peel<Pol>, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
+            hipLaunchKernelGGL((k_bq_peel<Pol, kPeelWaves<Pol>>), dim3(peel_blocks), dim3(256),
+                               0, stream, P, Q);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);
         }
         cur = nxt;
@@ -1468,7 +1474,8 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     hipLaunchKernelGGL(k_bq_finish<Pol>, dim3((unsigned)pl.bq_finish_blocks), dim3(256), 0,
                        stream, P, Q);
     if (Q.pr)
-        hipLaunchKernelGGL(k_bq_peel<Pol>, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
+        hipLaunchKernelGGL((k_bq_peel<Pol, kPeelWaves<Pol>>), dim3(peel_blocks), dim3(256), 0,
+                           stream, P, Q);
 }
 
 // Launches of a planned forward run on `stream`: n_chunks transport kernels
