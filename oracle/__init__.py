@@ -21,13 +21,23 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# KP_ORACLE_PLANT=k (k > 0) builds and loads the oracle with the k-th planted
+# mistake (kp_oracle.c, KP_PLANT) from KP_ORACLE_OUT, never over the real one
+PLANT = int(os.environ.get("KP_ORACLE_PLANT", "0") or 0)
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+if PLANT:
+    LIB_PATH = os.environ.get("KP_ORACLE_OUT", "")
+    if not LIB_PATH or os.path.dirname(os.path.abspath(LIB_PATH)) == HERE:
+        raise RuntimeError("KP_ORACLE_PLANT needs KP_ORACLE_OUT outside oracle/")
 
 _lib = None
 
 
 def build() -> str:
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    cmd = ["make", "-s", "-C", HERE]
+    if PLANT:
+        cmd += [f"PLANT={PLANT}", f"OUT={os.path.abspath(LIB_PATH)}"]
+    subprocess.run(cmd, check=True)
     return LIB_PATH
 
 

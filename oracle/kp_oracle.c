@@ -21,6 +21,14 @@
 
 #define TAU_CUT 40.0
 
+/* Planted mistakes: built with -DKP_PLANT=k (k > 0), the oracle carries the
+ * k-th deliberate error below, for the test that checks every one of them
+ * fails some pin.  The default build (KP_PLANT = 0) has none. */
+#ifndef KP_PLANT
+#define KP_PLANT 0
+#endif
+#define PLANT(k) (KP_PLANT == (k))
+
 /* ------------------------------------------------------------------ */
 /* Leaf wrappers                                                       */
 
@@ -92,7 +100,7 @@ static int get_dl(const ray_t *r, double *lmin)
     int ap = -1;
     double best = DBL_MAX;
     for (int a = 0; a < 3; ++a)
-        if (dl[a] >= 0.0 && dl[a] <= best) {
+        if (dl[a] >= 0.0 && (PLANT(10) ? dl[a] < best : dl[a] <= best)) {
             ap = a;
             best = dl[a];
         }
@@ -311,8 +319,8 @@ static void deposit(const fctx_t *c, fthr_t *t, int band, int o, const double *x
     double rel[3];
     for (int a = 0; a < 3; ++a)
         rel[a] = x[a] - c->center[a];
-    const double uh = dot3(rel, c->eh[o]);
-    const double uv = dot3(rel, c->ev[o]);
+    const double uh = dot3(rel, PLANT(8) ? c->ev[o] : c->eh[o]);
+    const double uv = dot3(rel, PLANT(8) ? c->eh[o] : c->ev[o]);
     const double fx = floor((uh + c->run->image_half) * c->pix_scale);
     const double fy = floor((uv + c->run->image_half) * c->pix_scale);
     if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)N && fy < (double)N))
@@ -330,15 +338,15 @@ static void phase(int model, double gg, double pmax, double mu, double *z11, dou
     const double m2 = mu * mu;
     if (model == KP_THOMSON) {
         *z11 = 0.75 * (1.0 + m2);
-        *z12 = 0.75 * (m2 - 1.0);
+        *z12 = PLANT(13) ? 0.75 * (1.0 - m2) : 0.75 * (m2 - 1.0);
         *z33 = 1.5 * mu;
     } else {
         const double D = (1.0 + gg * gg) - 2.0 * gg * mu;
-        const double ph = (1.0 - gg * gg) / (D * sqrt(D));
-        const double pol = pmax * (1.0 - m2) / (1.0 + m2);
+        const double ph = PLANT(6) ? (1.0 - gg * gg) / D : (1.0 - gg * gg) / (D * sqrt(D));
+        const double pol = PLANT(1) ? pmax * (1.0 - m2) : pmax * (1.0 - m2) / (1.0 + m2);
         *z11 = ph;
         *z12 = -pol * ph;
-        *z33 = ph * (2.0 * mu / (1.0 + m2));
+        *z33 = PLANT(2) ? ph * (2.0 * mu) : PLANT(3) ? ph * mu : ph * (2.0 * mu / (1.0 + m2));
     }
 }
 
@@ -389,7 +397,7 @@ static void peel_stokes(int model, double gg, double pmax, const double *d, cons
     const double s2p = 2.0 * cps * sps / nrm;
     Si[0] = Ip;
     Si[1] = Qp * c2p + Up * s2p;
-    Si[2] = Up * c2p - Qp * s2p;
+    Si[2] = PLANT(7) ? Qp * s2p - Up * c2p : Up * c2p - Qp * s2p;
 }
 
 /* Scattering by cos(theta)=mu and azimuth 2 pi phi_u measured from e1
@@ -404,12 +412,12 @@ static void scatter_stokes(int model, double pmax, double mu, double phi_u, doub
     kp_sincos2pi(phi_u, &sf, &cf);
     const double c2 = cf * cf - sf * sf;
     const double s2 = 2.0 * cf * sf;
-    const double Qr = S[1] * c2 + S[2] * s2;
-    const double Ur = S[2] * c2 - S[1] * s2;
+    const double Qr = PLANT(9) ? S[1] * c2 - S[2] * s2 : S[1] * c2 + S[2] * s2;
+    const double Ur = PLANT(9) ? S[2] * c2 + S[1] * s2 : S[2] * c2 - S[1] * s2;
     const double m2 = mu * mu;
     const double pl = (model == KP_THOMSON) ? 1.0 : pmax;
-    const double a12 = -pl * (1.0 - m2) / (1.0 + m2);
-    const double a33 = 2.0 * mu / (1.0 + m2);
+    const double a12 = (PLANT(5) ? pl : -pl) * (1.0 - m2) / (1.0 + m2);
+    const double a33 = (PLANT(4) ? pl : 1.0) * (2.0 * mu / (1.0 + m2));
     const double In = S[0] + a12 * Qr;
     const double Qn = a12 * S[0] + Qr;
     const double Un = a33 * Ur;
@@ -525,7 +533,7 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
                 if (li > l)
                     li = l;
                 if (t->J)
-                    t->J[gc * nbands + band] += I * (li / V);
+                    t->J[gc * nbands + band] += I * (PLANT(11) ? li : li / V);
                 for (int a = 0; a < 3; ++a)
                     r.x[a] = r.x[a] + li * r.d[a];
                 interact = 1;
@@ -533,7 +541,7 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
             }
             tau_rem = tau_rem - dtau;
             if (t->J)
-                t->J[gc * nbands + band] += I * (l / V);
+                t->J[gc * nbands + band] += I * (PLANT(11) ? l : l / V);
             int reload;
             const int st = cross_face(m, &r, ap, l, &reload);
             if (st == ST_ESCAPED) {
@@ -555,7 +563,8 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
         uint32_t u[4] = {plo, phi, (uint32_t)events, 0u};
         kp_philox4x32_10(u, k0, k1);
         const size_t gc = (size_t)r.ib * n3 + cell;
-        const double alb = (bd->flags & KP_BAND_CELL_ALBEDO) ? (double)c->albedo[gc] : bd->albedo;
+        const double alb = PLANT(12) ? 1.0
+                         : (bd->flags & KP_BAND_CELL_ALBEDO) ? (double)c->albedo[gc] : bd->albedo;
         const double gg = (bd->flags & KP_BAND_CELL_G) ? (c->g ? (double)c->g[gc] : 0.0) : bd->g;
         I = I * alb;
         Q = Q * alb;
@@ -570,7 +579,7 @@ static void packet(const fctx_t *c, fthr_t *t, int band, uint64_t pid, uint64_t 
             peel_stokes(run->scatter_model, gg, run->p_max, r.d, e1, e2, S, c->no[o], c->ev[o],
                         Si);
             const double tau = tau_to_edge(c, &r, c->no[o], ks, &npeel);
-            const double att = kp_exp(-tau) * KP_INV_4PI;
+            const double att = PLANT(14) ? KP_INV_4PI : kp_exp(-tau) * KP_INV_4PI;
             deposit(c, t, band, o, r.x, Si[0] * att, Si[1] * att, Si[2] * att);
         }
 
@@ -780,8 +789,8 @@ static long pix_index(const fctx_t *c, int band, int o, const double *x)
     double rel[3];
     for (int a = 0; a < 3; ++a)
         rel[a] = x[a] - c->center[a];
-    const double uh = dot3(rel, c->eh[o]);
-    const double uv = dot3(rel, c->ev[o]);
+    const double uh = dot3(rel, PLANT(8) ? c->ev[o] : c->eh[o]);
+    const double uv = dot3(rel, PLANT(8) ? c->eh[o] : c->ev[o]);
     const double fx = floor((uh + c->run->image_half) * c->pix_scale);
     const double fy = floor((uv + c->run->image_half) * c->pix_scale);
     if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)N && fy < (double)N))
