@@ -108,6 +108,7 @@ template <class Pol> struct qlane {
     double tau;
     uint64_t slot, pid;
     int band, events, steps, status;
+    int blk; // MARCH_BLOCK: exit axis + 4 * (exit through the upper face)
     uint32_t ncross, npeel;
 };
 
@@ -327,19 +328,25 @@ __device__ bool interact(const kparams &P, const qparams &Qp, qlane<Pol> &L, int
     return true;
 }
 
-enum : int { MARCH_END = 0, MARCH_OUT = 1 };
+enum : int { MARCH_END = 0, MARCH_OUT = 1, MARCH_BLOCK = 2 };
 
 // The packet's march from its current point.  LOCAL: kappa_rho from the LDS
 // tile lk of the packet's block and J into the LDS tile lj; returns
-// MARCH_OUT as soon as the ray has crossed into another block (the lane then
-// holds it in the new block), or after Q.yield steps and interactions in
-// this block (it goes back to the queue where it stands).  Otherwise kappa_rho from HBM and J by global
+// MARCH_BLOCK as soon as the ray has reached its block's edge (leave_block
+// then takes it into the next block: a wave's lanes do that together once
+// all are done with this block), or MARCH_OUT after Q.yield steps and
+// interactions in this block (it goes back to the queue where it stands).  Otherwise kappa_rho from HBM and J by global
 // atomics, to the packet's end.  MARCH_END: L.status holds the end status.
 // J tile of a block in LDS: FP64 for the FP64 path; FP32 for the mixed path,
 // whose per-task sums (a few hundred contributions per cell at most) keep
 // FP32's 7 digits, far below the Monte Carlo noise, before the FP64 atomic
 // flush.  Half the LDS lets more workgroups share a CU.
 template <class Pol> using tile_t = typename Pol::real;
+
+// march<LOCAL> leaves the block change to leave_block, after the wave's
+// lanes are done with the block: the block-change code then runs once per
+// wave and march instead of in nearly every step.
+template <class Pol> constexpr bool kQueueDeferBlock = true;
 
 template <class Pol, bool LOCAL>
 __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
@@ -406,6 +413,10 @@ __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const fl
             int st = Pol::cross_step(P, L.r, ap, l);
             if (st == f32::ST_IN)
                 continue;
+            if constexpr (LOCAL && kQueueDeferBlock<Pol>) {
+                L.blk = ap + (st == f32::ST_BLOCK_HI ? 4 : 0);
+                return MARCH_BLOCK;
+            }
             st = Pol::change_block(P, L.r, ap, st == f32::ST_BLOCK_HI);
             if (st == f32::ST_ESCAPED) {
                 L.status = KP_END_ESCAPED;
@@ -427,22 +438,47 @@ __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const fl
     }
 }
 
+// The block change of a lane that march<LOCAL> left at its block's edge:
+// MARCH_OUT with the packet in the next block, or MARCH_END (escaped through
+// a domain face, or lost).
+template <class Pol> __device__ __forceinline__ int leave_block(const kparams &P, qlane<Pol> &L)
+{
+    const int st = Pol::change_block(P, L.r, L.blk & 3, L.blk >> 2);
+    if (st == f32::ST_IN)
+        return MARCH_OUT;
+    L.status = (st == f32::ST_ESCAPED) ? KP_END_ESCAPED : KP_END_LOST;
+    return MARCH_END;
+}
+
+// Counters of a workgroup in LDS (KP_C_* order): a packet's end adds to them
+// with LDS atomics, and the workgroup adds them to the launch's counters
+// once (wg_counters_flush).  Kept out of the lanes' registers, which the
+// march needs.
+__device__ __forceinline__ void wg_counters_zero(unsigned long long *sc)
+{
+    if (threadIdx.x < KP_NCOUNTERS)
+        sc[threadIdx.x] = 0;
+}
+__device__ __forceinline__ void wg_counters_flush(const kparams &P, const unsigned long long *sc)
+{
+    if (P.counters && threadIdx.x < KP_NCOUNTERS && sc[threadIdx.x] != 0)
+        atomicAdd(P.counters + threadIdx.x, sc[threadIdx.x]);
+}
+
 // Counters and end state of a packet that ended.
 template <class Pol>
-__device__ __forceinline__ void end_packet(const kparams &P, const qlane<Pol> &L, lane_counters &C)
+__device__ __forceinline__ void end_packet(const kparams &P, const qlane<Pol> &L,
+                                           unsigned long long *sc)
 {
-    C.cross += L.ncross;
-    C.peel += L.npeel;
-    C.events += (uint32_t)L.events;
-    C.packets += 1;
-    if (L.status == KP_END_ESCAPED)
-        C.escaped += 1;
-    else if (L.status == KP_END_RR)
-        C.rr += 1;
-    else if (L.status == KP_END_STEPS)
-        C.steps += 1;
-    else
-        C.lost += 1;
+    atomicAdd(&sc[KP_C_CROSS_MAIN], (unsigned long long)L.ncross);
+    atomicAdd(&sc[KP_C_CROSS_PEEL], (unsigned long long)L.npeel);
+    atomicAdd(&sc[KP_C_EVENTS], (unsigned long long)L.events);
+    atomicAdd(&sc[KP_C_PACKETS], 1ull);
+    const int k = L.status == KP_END_ESCAPED ? KP_C_ESCAPED
+                  : L.status == KP_END_RR    ? KP_C_KILLED_RR
+                  : L.status == KP_END_STEPS ? KP_C_KILLED_STEPS
+                                             : KP_C_LOST;
+    atomicAdd(&sc[k], 1ull);
     if (P.end_state) {
         double *es = P.end_state + L.slot * KP_ES_STRIDE;
         double g[3];
@@ -495,7 +531,7 @@ __device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<t
             double Ip, Qi, Ui;
             f64::peel_stokes(P.model, q.g, P.pmax, q.d, q.e1, e2, q.s[0], q.s[1], q.s[2],
                              P.no[o], P.ev[o], Ip, Qi, Ui);
-            const double tau = f64::tau_to_edge(P, t, P.no[o], ks, np);
+            const double tau = f64::tau_to_edge_sync(P, t, P.no[o], ks, np);
             const double att = kp_exp(-tau) * KP_INV_4PI;
             f64::deposit(P, band, o, t.x, Ip * att, Qi * att, Ui * att);
         }
@@ -510,7 +546,7 @@ __device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<t
             float Ip, Qi, Ui;
             f32::peel_stokes(P.model, q.g, (float)P.pmax, q.d, q.e1, e2, q.s[0], q.s[1],
                              q.s[2], no, ev, Ip, Qi, Ui);
-            const double tau = f32::tau_to_edge(P, t, no, ks, np);
+            const double tau = f32::tau_to_edge_sync(P, t, no, ks, np);
             const float att = f32::fast_exp(-(float)tau) * f32::kInv4Pi;
             f32::deposit(P, band, o, gx, Ip * att, Qi * att, Ui * att);
         }

@@ -529,7 +529,8 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     float *lk = reinterpret_cast<float *>(lj + P.n3);
     __shared__ uint32_t s_key;
     __shared__ uint64_t s_end;
-    kp::lane_counters C;
+    __shared__ unsigned long long s_cnt[KP_NCOUNTERS];
+    kp::bq::wg_counters_zero(s_cnt);
     const uint64_t n = *Q.n_in;
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
     kp::bq::wave_peel_open();
@@ -561,10 +562,13 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
             for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
                 kp::bq::qlane<Pol> L;
                 kp::bq::load<Pol>(P, in[Q.order[q]], L);
-                if (kp::bq::march<Pol, true>(P, Q, L, lk, lj) == kp::bq::MARCH_OUT)
+                int m = kp::bq::march<Pol, true>(P, Q, L, lk, lj);
+                if (m == kp::bq::MARCH_BLOCK)
+                    m = kp::bq::leave_block<Pol>(P, L);
+                if (m == kp::bq::MARCH_OUT)
                     kp::bq::push_out<Pol>(P, Q, L);
                 else
-                    kp::bq::end_packet<Pol>(P, L, C);
+                    kp::bq::end_packet<Pol>(P, L, s_cnt);
             }
             __syncthreads();
             if (P.J) {
@@ -580,7 +584,8 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
         }
     }
     kp::bq::wave_peel_close<real>(Q);
-    flush_counters(P, C);
+    __syncthreads();
+    kp::bq::wg_counters_flush(P, s_cnt);
 }
 
 // The peel-offs queued by a pass: one interaction per lane, in queue order,
@@ -605,7 +610,9 @@ template <class Pol>
 __global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparams Q)
 {
     using real = typename Pol::real;
-    kp::lane_counters C;
+    __shared__ unsigned long long s_cnt[KP_NCOUNTERS];
+    kp::bq::wg_counters_zero(s_cnt);
+    __syncthreads();
     const uint64_t n = *Q.n_in;
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -614,10 +621,11 @@ __global__ void __launch_bounds__(256) k_bq_finish(kp::kparams P, kp::bq::qparam
         kp::bq::qlane<Pol> L;
         kp::bq::load<Pol>(P, in[k], L);
         (void)kp::bq::march<Pol, false>(P, Q, L, nullptr, nullptr);
-        kp::bq::end_packet<Pol>(P, L, C);
+        kp::bq::end_packet<Pol>(P, L, s_cnt);
     }
     kp::bq::wave_peel_close<real>(Q);
-    flush_counters(P, C);
+    __syncthreads();
+    kp::bq::wg_counters_flush(P, s_cnt);
 }
 
 // Sort key of every queued peel-off ray: observer, then the Morton code of
@@ -1398,10 +1406,11 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
     return pl;
 }
 
-// Waves per SIMD the peel kernel is compiled for: the mixed kernel's march is
-// bound by kappa_rho load latency, so it keeps 8 waves per SIMD resident (64
-// VGPRs); the FP64 one runs at its natural register count.
-template <class Pol> constexpr int kPeelWaves = sizeof(typename Pol::real) == 4 ? 8 : 1;
+// Waves per SIMD the peel kernel is compiled for: the mixed kernel keeps 4
+// waves per SIMD resident (up to 128 VGPRs, no spills: at 8 waves and 64
+// VGPRs it spilled and ran 3 % slower on config 2); the FP64 one runs at its
+// natural register count.
+template <class Pol> constexpr int kPeelWaves = sizeof(typename Pol::real) == 4 ? 4 : 1;
 
 // The block-queue part of a forward run (kernels only): emission peel-offs
 // and records, queue_passes passes (sort by key, segment table, pass), then
@@ -1456,8 +1465,8 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
         Q.out_key = fb.bq_key[nxt];
         Q.n_out = fb.bq_cnt + nxt;
         // the mixed pass kernel is held to 4 waves per SIMD (128 VGPRs), the
-        // FP64 one runs at its natural register count
-        constexpr int wps = sizeof(typename Pol::real) == 4 ? 4 : 1;
+        // FP64 one to 2 (256)
+        constexpr int wps = sizeof(typename Pol::real) == 4 ? 4 : 2;
         hipLaunchKernelGGL((k_bq_pass<Pol, wps>), dim3((unsigned)pl.bq_pass_blocks), dim3(256),
                            (size_t)P.n3 * (sizeof(kp::bq::tile_t<Pol>) + sizeof(float)),
                            stream, P, Q);

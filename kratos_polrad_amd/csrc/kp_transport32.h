@@ -105,10 +105,12 @@ __device__ __forceinline__ int cross_in_block(const kparams &P, ray_t &r, int ap
             r.x[a] = fmaf(l, r.cpl[a], r.x[a]);
         }
     }
-    // runtime axis indices are resolved with unrolled selects so the ray
-    // stays in registers
-    const int ia = ap == 0 ? r.i[0] : (ap == 1 ? r.i[1] : r.i[2]);
-    return (ia >= 0 && ia < n) ? ST_IN : (ia > 0 ? ST_BLOCK_HI : ST_BLOCK_LO);
+    // only the exit axis' index moved, so it is the one that can have left
+    // [0,n): tested on all three axes, without selecting the axis (a select
+    // chain on ap compiles to branches)
+    const bool hi = (r.i[0] >= n) | (r.i[1] >= n) | (r.i[2] >= n);
+    const bool lo = (r.i[0] < 0) | (r.i[1] < 0) | (r.i[2] < 0);
+    return hi ? ST_BLOCK_HI : (lo ? ST_BLOCK_LO : ST_IN);
 }
 
 // Block change of a ray whose index left [0,n) through the exit face of axis
@@ -177,6 +179,15 @@ __device__ __forceinline__ int cell_index(const kparams &P, const ray_t &r)
     return r.i[0] + P.n_cell * (r.i[1] + P.n_cell * r.i[2]);
 }
 
+// cell_index with the indices clamped into the block: the cell itself while
+// the ray is in its block, a cell of the block after it stepped out.
+__device__ __forceinline__ int cell_index_clamped(const kparams &P, const ray_t &r)
+{
+    const int m = P.n_cell - 1;
+    return min(max(r.i[0], 0), m) +
+           P.n_cell * (min(max(r.i[1], 0), m) + P.n_cell * min(max(r.i[2], 0), m));
+}
+
 __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *dir, float ks,
                               uint64_t &ncross)
 {
@@ -203,6 +214,57 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *
         ncross += 1;
         if (tau > kTauCut || st != ST_IN)
             break;
+    }
+    return tau;
+}
+
+// tau_to_edge with the block changes of a wave's rays taken together: each
+// lane marches to the edge of its block (or its end), and the rays that left
+// their block change it once every ray of the wave has left its block or
+// ended.  Parallel rays of one wave that start in one block then enter the
+// following blocks together, and the block-change code runs once per block
+// instead of in nearly every step.  Per ray the operations and their order
+// are those of tau_to_edge, so the result is the same.
+__device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const float *dir,
+                                   float ks, uint64_t &ncross)
+{
+    ray_t r = from;
+    r.d[0] = dir[0];
+    r.d[1] = dir[1];
+    r.d[2] = dir[2];
+    set_scales(P, r);
+    double tau = 0.0;
+    float kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
+    for (;;) {
+        int ap, st;
+        for (;;) {
+            float l;
+            ap = get_dl(r, l);
+            if (ap < 0) {
+                st = ST_LOST;
+                break;
+            }
+            const float kap = kc * ks;
+            st = cross_in_block(P, r, ap, l);
+            // issued whether or not the ray is still in the block (then the
+            // value is not used), so kc is one register loaded one step
+            // ahead: a conditional load here made the compiler copy it into
+            // place, waiting for the load within the step
+            kc = P.kappa[(size_t)r.ib * P.n3 + cell_index_clamped(P, r)];
+            tau += (double)(kap * l);
+            ncross += 1;
+            if (tau > kTauCut) {
+                st = ST_LOST;
+                break;
+            }
+            if (st != ST_IN)
+                break;
+        }
+        if (st != ST_BLOCK_LO && st != ST_BLOCK_HI)
+            break;
+        if (change_block(P, r, ap, st == ST_BLOCK_HI) != ST_IN)
+            break;
+        kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
     }
     return tau;
 }

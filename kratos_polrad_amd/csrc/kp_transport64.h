@@ -126,11 +126,13 @@ __device__ __forceinline__ int cross_in_block(const kparams &P, ray_t &r, int ap
         }
     }
     const int n = P.n_cell;
-    // runtime axis indices are resolved with unrolled selects so the ray
-    // stays in registers
-    const int ia = ap == 0 ? r.i[0] : (ap == 1 ? r.i[1] : r.i[2]);
-    if (!(ia >= 0 && ia < n))
-        return ia > 0 ? ST_BLOCK_HI : ST_BLOCK_LO;
+    // only the exit axis' index moved, so it is the one that can have left
+    // [0,n): tested on all three axes, without selecting the axis (a select
+    // chain on ap compiles to branches)
+    const bool hi = (r.i[0] >= n) | (r.i[1] >= n) | (r.i[2] >= n);
+    const bool lo = (r.i[0] < 0) | (r.i[1] < 0) | (r.i[2] < 0);
+    if (hi | lo)
+        return hi ? ST_BLOCK_HI : ST_BLOCK_LO;
     regulate(P, r, ap, reload);
     return ST_IN;
 }
@@ -197,6 +199,14 @@ __device__ __forceinline__ int cell_index(const kparams &P, const ray_t &r)
     return r.i[0] + P.n_cell * (r.i[1] + P.n_cell * r.i[2]);
 }
 
+// cell_index with the indices clamped into the block (see f32).
+__device__ __forceinline__ int cell_index_clamped(const kparams &P, const ray_t &r)
+{
+    const int m = P.n_cell - 1;
+    return min(max(r.i[0], 0), m) +
+           P.n_cell * (min(max(r.i[1], 0), m) + P.n_cell * min(max(r.i[2], 0), m));
+}
+
 __device__ double tau_to_edge(const kparams &P, const ray_t &from, const double *dir, double ks,
                               uint64_t &ncross)
 {
@@ -226,6 +236,54 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const double 
         ncross += 1;
         if (tau > kTauCut || st != ST_IN)
             break;
+    }
+    return tau;
+}
+
+// tau_to_edge with the block changes of a wave's rays taken together (see
+// f32::tau_to_edge_sync); same operations in the same order per ray.
+__device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const double *dir,
+                                   double ks, uint64_t &ncross)
+{
+    ray_t r = from;
+    r.d[0] = dir[0];
+    r.d[1] = dir[1];
+    r.d[2] = dir[2];
+    double tau = 0.0;
+    float kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
+    for (;;) {
+        int ap, st;
+        for (;;) {
+            double l;
+            ap = get_dl(r, l);
+            if (ap < 0) {
+                st = ST_LOST;
+                break;
+            }
+            const double kap = (double)kc * ks;
+            bool reload;
+            st = cross_in_block(P, r, ap, l, reload);
+            // issued whether or not the ray is still in the block (see f32)
+            kc = P.kappa[(size_t)r.ib * P.n3 + cell_index_clamped(P, r)];
+            if (st == ST_IN && reload)
+                gl_load(P, r);
+            tau = tau + kap * l;
+            ncross += 1;
+            if (tau > kTauCut) {
+                st = ST_LOST;
+                break;
+            }
+            if (st != ST_IN)
+                break;
+        }
+        if (st != ST_BLOCK_LO && st != ST_BLOCK_HI)
+            break;
+        bool reload = false;
+        if (change_block(P, r, ap, st == ST_BLOCK_HI, reload) != ST_IN)
+            break;
+        if (reload)
+            gl_load(P, r);
+        kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
     }
     return tau;
 }
