@@ -108,7 +108,10 @@ template <class Pol> struct qlane {
     double tau;
     uint64_t slot, pid;
     int band, events, steps, status;
-    int blk; // MARCH_BLOCK: exit axis + 4 * (exit through the upper face)
+    int blk;       // MARCH_BLOCK: exit axis + 4 * (exit through the upper face)
+    int ret;       // pass kernel: how the lane's march in the block ended (MARCH_*)
+    int cell;      // pass kernel: cell of the pending interaction
+    uint32_t work; // pass kernel: steps and interactions in this pass
     uint32_t ncross, npeel;
 };
 
@@ -438,12 +441,93 @@ __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const fl
     }
 }
 
+// Modes of a lane in the pass kernel's wave loop (pass_block).
+enum : int { Q_IDLE = 0, Q_MARCH = 1, Q_INT = 2, Q_RETIRE = 3 };
+// A wave runs the interactions of its lanes once this many wait (or no lane
+// marches), and retires and refills lanes once this many are done.
+constexpr int kPassIntBatch = 16;
+constexpr int kPassRetireBatch = 32;
+// Whether the pass kernel runs its records in the wave loop (pass_block) or
+// one record per lane at a time (march<true>, each wave waiting for its
+// slowest lane).  The FP64 step is costly enough that keeping lanes busy
+// wins (config 2, 2e7 packets: 213 against 280 ms); the mixed step is cheap,
+// and the wave loop's phases cost more than the idle lanes (121-141 against
+// 108 ms over batch sizes 16-64).
+template <class Pol> constexpr bool kPassWaveLoop = sizeof(typename Pol::real) == 8;
+
+// One step of a marching lane in its block (kappa_rho from the LDS tile lk,
+// J into the LDS tile lj): the step of march<true>, ending in a mode instead
+// of a loop exit.  Q_INT: the interaction point is reached (cell in L.cell);
+// Q_RETIRE: the lane's march in this pass ends (L.ret).
+template <class Pol>
+__device__ __forceinline__ int pass_step(const kparams &P, const qparams &Q, qlane<Pol> &L,
+                                         const float *lk, tile_t<Pol> *lj)
+{
+    using real = typename Pol::real;
+    if (L.work >= Q.yield) {
+        L.ret = MARCH_OUT;
+        return Q_RETIRE;
+    }
+    ++L.work;
+    if (L.steps <= 0) {
+        L.status = KP_END_STEPS;
+        L.ret = MARCH_END;
+        return Q_RETIRE;
+    }
+    real l;
+    const int ap = Pol::dl(L.r, l);
+    if (ap < 0) {
+        L.status = KP_END_LOST;
+        L.ret = MARCH_END;
+        return Q_RETIRE;
+    }
+    const int cell = Pol::cell(P, L.r);
+    const real kap = (real)lk[cell] * L.ks;
+    const real dtau = kap * l;
+    L.ncross += 1;
+    L.steps -= 1;
+    bool hit;
+    real seg;
+    if constexpr (sizeof(real) == 8) {
+        hit = kap > 0.0 && dtau >= L.tau;
+        seg = l;
+        if (hit) {
+            const double li = L.tau / kap;
+            seg = li > l ? l : li;
+        }
+    } else {
+        hit = kap > 0.0f && (double)dtau >= L.tau;
+        seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
+    }
+    if (P.J) {
+        real v;
+        if constexpr (sizeof(real) == 8)
+            v = L.I * (seg / P.vol0[L.r.ib]);
+        else
+            v = L.I * seg * L.r.iv;
+        atomicAdd(&lj[cell], v);
+    }
+    if (hit) {
+        Pol::advance(L.r, seg);
+        L.cell = cell;
+        return Q_INT;
+    }
+    L.tau = L.tau - (double)dtau;
+    const int st = Pol::cross_step(P, L.r, ap, l);
+    if (st == f32::ST_IN)
+        return Q_MARCH;
+    L.blk = ap + (st == f32::ST_BLOCK_HI ? 4 : 0);
+    L.ret = MARCH_BLOCK;
+    return Q_RETIRE;
+}
+
 // The block change of a lane that march<LOCAL> left at its block's edge:
 // MARCH_OUT with the packet in the next block, or MARCH_END (escaped through
 // a domain face, or lost).
-template <class Pol> __device__ __forceinline__ int leave_block(const kparams &P, qlane<Pol> &L)
+template <class Pol, class NB = nb_global>
+__device__ __forceinline__ int leave_block(const kparams &P, qlane<Pol> &L, const NB &T)
 {
-    const int st = Pol::change_block(P, L.r, L.blk & 3, L.blk >> 2);
+    const int st = Pol::change_block(P, L.r, L.blk & 3, L.blk >> 2, T);
     if (st == f32::ST_IN)
         return MARCH_OUT;
     L.status = (st == f32::ST_ESCAPED) ? KP_END_ESCAPED : KP_END_LOST;
@@ -554,6 +638,104 @@ __device__ __forceinline__ uint32_t peel_ray(const kparams &P, const ray_rec_t<t
     if (P.end_state)
         atomicAdd(&P.end_state[(uint64_t)q.slot * KP_ES_STRIDE + 14], (double)np);
     return (uint32_t)np;
+}
+
+// The records of one block segment of a pass, sorted positions p .. e-1,
+// marched by the workgroup's waves with the block's kappa_rho in lk, its J
+// tile in lj and its face-neighbour entries in nbt.  Each lane holds one packet at a time; per loop iteration every
+// marching lane takes one step, and the wave runs
+//   - the interactions of its waiting lanes together once kPassIntBatch
+//     lanes wait (or none marches),
+//   - the ends of its lanes' marches together (block change and record of
+//     the next pass, or the packet's end), each lane then taking the next
+//     record of the segment, once kPassRetireBatch lanes are done (or none
+//     marches); a wave claims the segment's sorted positions 64 at a time
+//     (*s_next: the next unclaimed one, shared by the workgroup's waves) and
+//     holds their record indices one per lane, so a refill waits for one
+//     load (the record), not two,
+// so lanes do not wait for the wave's slowest packet.  Every packet runs the
+// steps and interactions of march<true>, in the same order.
+template <class Pol>
+__device__ void pass_block(const kparams &P, const qparams &Q,
+                           const rec_t<typename Pol::real> *in, uint64_t p, uint64_t e,
+                           const float *lk, tile_t<Pol> *lj, const blk_nb_t *nbt,
+                           unsigned long long *sc, uint32_t *s_next)
+{
+    const uint32_t n = (uint32_t)(e - p);
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    qlane<Pol> L;
+    int mode = Q_IDLE;
+    bool more = true; // wave-uniform: the segment may still hold records
+    // the wave's window of claimed positions (wave-uniform): wlen of them
+    // from wpos, wused taken; lane k holds the record index of wpos + k
+    uint32_t wpos = 0, wlen = 0, wused = 0;
+    int32_t word = 0;
+    for (;;) {
+        if (mode == Q_MARCH)
+            mode = pass_step<Pol>(P, Q, L, lk, lj);
+        const int nm = __popcll(__ballot(mode == Q_MARCH));
+        const int ni = __popcll(__ballot(mode == Q_INT));
+        if (ni >= kPassIntBatch || (ni > 0 && nm == 0)) {
+            if (mode == Q_INT) {
+                if (interact<Pol>(P, Q, L, (int64_t)L.r.ib * P.n3 + L.cell)) {
+                    ++L.work;
+                    mode = Q_MARCH;
+                } else {
+                    L.status = KP_END_RR;
+                    L.ret = MARCH_END;
+                    mode = Q_RETIRE;
+                }
+            }
+            continue;
+        }
+        const int nr = __popcll(__ballot(mode == Q_RETIRE));
+        const int nfree = 64 - nm - ni; // retiring and idle lanes
+        if (nr >= kPassRetireBatch || (more && nfree >= kPassRetireBatch) || nm == 0) {
+            if (mode == Q_RETIRE) {
+                int m = L.ret;
+                if (m == MARCH_BLOCK)
+                    m = leave_block<Pol>(P, L, nb_local{nbt});
+                if (m == MARCH_OUT)
+                    push_out<Pol>(P, Q, L);
+                else
+                    end_packet<Pol>(P, L, sc);
+                mode = Q_IDLE;
+            }
+            for (uint64_t idle = __ballot(mode == Q_IDLE); more && idle;
+                 idle = __ballot(mode == Q_IDLE)) {
+                if (wused == wlen) { // a new window of positions
+                    uint32_t b = 0;
+                    if (lane == 0)
+                        b = atomicAdd(s_next, 64u);
+                    b = __shfl(b, 0, 64);
+                    if (b >= n) {
+                        more = false;
+                        break;
+                    }
+                    wpos = b;
+                    wlen = n - b < 64u ? n - b : 64u;
+                    wused = 0;
+                    word = (uint32_t)lane < wlen ? Q.order[p + b + lane] : 0;
+                }
+                const uint32_t nidle = (uint32_t)__popcll(idle);
+                const uint32_t take = nidle < wlen - wused ? nidle : wlen - wused;
+                const uint32_t rk = (uint32_t)__popcll(idle & below);
+                const int src = (int)(wused + (rk < take ? rk : take - 1));
+                const int32_t o = __shfl(word, src, 64);
+                if (mode == Q_IDLE && rk < take) {
+                    load<Pol>(P, in[o], L);
+                    L.work = 0;
+                    mode = Q_MARCH;
+                }
+                wused += take;
+                if (wused == wlen && wpos + wlen >= n)
+                    more = false;
+            }
+            if (!more && __ballot(mode != Q_IDLE) == 0)
+                break;
+        }
+    }
 }
 
 // Record of the next pass for each calling lane that crossed into another

@@ -141,6 +141,45 @@ struct kparams {
     const unsigned int *total_dev; // device-side claim bound (or null)
 };
 
+// The face-neighbour entries of a block that a block change reads (the
+// proxy's neighbors / d_lvl / physical-boundary bits): from the engine's
+// block table (nb_global), or from a copy of one block's entries in LDS
+// (nb_local) where every lane is in that block.
+struct nb_global {
+    const kparams &P;
+    __device__ uint32_t phy(int ib) const { return P.phybnd[ib]; }
+    __device__ int dl(int ib, int k) const { return P.d_lvl[6 * ib + k]; }
+    __device__ int id(int ib, int k) const { return P.nb_id[24 * ib + k]; }
+    __device__ int shift(int ib, int k) const { return P.nb_shift[24 * ib + k]; }
+};
+struct blk_nb_t {
+    int32_t id[24];
+    uint8_t shift[24];
+    int8_t dl[6];
+    uint8_t phy;
+};
+struct nb_local {
+    const blk_nb_t *t;
+    __device__ uint32_t phy(int) const { return t->phy; }
+    __device__ int dl(int, int k) const { return t->dl[k]; }
+    __device__ int id(int, int k) const { return t->id[k]; }
+    __device__ int shift(int, int k) const { return t->shift[k]; }
+};
+// Copies block ib's entries into t (threads 0..23 of the workgroup; the
+// caller synchronises before use).
+__device__ __forceinline__ void stage_nb(const kparams &P, int ib, blk_nb_t *t)
+{
+    const int k = threadIdx.x;
+    if (k < 24) {
+        t->id[k] = P.nb_id[24 * ib + k];
+        t->shift[k] = P.nb_shift[24 * ib + k];
+    }
+    if (k < 6)
+        t->dl[k] = P.d_lvl[6 * ib + k];
+    if (k == 0)
+        t->phy = P.phybnd[ib];
+}
+
 __device__ __forceinline__ double *g_base(const kparams &P, double *base)
 {
     return base + (uint64_t)(blockIdx.x % P.g_nrep) * P.g_stride;

@@ -519,7 +519,9 @@ __global__ void __launch_bounds__(256) k_bq_seg(const uint32_t *skey,
 
 // One pass of the block queue: tasks of `chunk` sorted records; per block of
 // a task the block's kappa_rho and a J tile in LDS, each packet marched to
-// the block's edge or its end, the tile added to J.
+// the block's edge or its end (in the FP64 path by the wave loop pass_block,
+// with the block's face-neighbour entries in LDS too; in the mixed path one
+// record per lane at a time), the tile added to J.
 template <class Pol, int WPS>
 __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qparams Q)
 {
@@ -529,6 +531,8 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     float *lk = reinterpret_cast<float *>(lj + P.n3);
     __shared__ uint32_t s_key;
     __shared__ uint64_t s_end;
+    __shared__ uint32_t s_next;
+    __shared__ kp::blk_nb_t s_nb;
     __shared__ unsigned long long s_cnt[KP_NCOUNTERS];
     kp::bq::wg_counters_zero(s_cnt);
     const uint64_t n = *Q.n_in;
@@ -547,6 +551,7 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
                     e = p1;
                 s_key = key;
                 s_end = e;
+                s_next = 0;
             }
             __syncthreads();
             const uint32_t key = s_key;
@@ -558,17 +563,23 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
                 lk[c] = kb[c];
                 lj[c] = (tile)0;
             }
+            if constexpr (kp::bq::kPassWaveLoop<Pol>)
+                kp::stage_nb(P, ib, &s_nb);
             __syncthreads();
-            for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
-                kp::bq::qlane<Pol> L;
-                kp::bq::load<Pol>(P, in[Q.order[q]], L);
-                int m = kp::bq::march<Pol, true>(P, Q, L, lk, lj);
-                if (m == kp::bq::MARCH_BLOCK)
-                    m = kp::bq::leave_block<Pol>(P, L);
-                if (m == kp::bq::MARCH_OUT)
-                    kp::bq::push_out<Pol>(P, Q, L);
-                else
-                    kp::bq::end_packet<Pol>(P, L, s_cnt);
+            if constexpr (kp::bq::kPassWaveLoop<Pol>) {
+                kp::bq::pass_block<Pol>(P, Q, in, p, e, lk, lj, &s_nb, s_cnt, &s_next);
+            } else {
+                for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
+                    kp::bq::qlane<Pol> L;
+                    kp::bq::load<Pol>(P, in[Q.order[q]], L);
+                    int m = kp::bq::march<Pol, true>(P, Q, L, lk, lj);
+                    if (m == kp::bq::MARCH_BLOCK)
+                        m = kp::bq::leave_block<Pol>(P, L, kp::nb_global{P});
+                    if (m == kp::bq::MARCH_OUT)
+                        kp::bq::push_out<Pol>(P, Q, L);
+                    else
+                        kp::bq::end_packet<Pol>(P, L, s_cnt);
+                }
             }
             __syncthreads();
             if (P.J) {

@@ -84,13 +84,18 @@ struct pol64 {
             f64::gl_load(P, r);
         return st;
     }
-    __device__ static int change_block(const kparams &P, ray &r, int a, int side)
+    template <class NB>
+    __device__ static int change_block(const kparams &P, ray &r, int a, int side, const NB &T)
     {
         bool rl = false;
-        const int st = f64::change_block(P, r, a, side, rl);
+        const int st = f64::change_block(P, r, a, side, rl, T);
         if (st == f64::ST_IN && rl)
             f64::gl_load(P, r);
         return st;
+    }
+    __device__ static int change_block(const kparams &P, ray &r, int a, int side)
+    {
+        return change_block(P, r, a, side, nb_global{P});
     }
     __device__ static void advance(ray &r, real l)
     {
@@ -138,6 +143,11 @@ struct pol32 {
     __device__ static int cross_step(const kparams &P, ray &r, int ap, real l)
     {
         return f32::cross_in_block(P, r, ap, l);
+    }
+    template <class NB>
+    __device__ static int change_block(const kparams &P, ray &r, int a, int side, const NB &T)
+    {
+        return f32::change_block(P, r, a, side, T);
     }
     __device__ static int change_block(const kparams &P, ray &r, int a, int side)
     {

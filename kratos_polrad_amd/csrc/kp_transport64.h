@@ -139,14 +139,15 @@ __device__ __forceinline__ int cross_in_block(const kparams &P, ray_t &r, int ap
 
 // Block change of a ray whose index left [0,n) through the exit face of axis
 // a (side 1: the upper face), then the periodic wrap; reload is set.
+template <class NB>
 __device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, int side,
-                                            bool &reload)
+                                            bool &reload, const NB &T)
 {
     const int n = P.n_cell;
-    if (P.phybnd[r.ib] & (1u << (2 * a + side)))
+    if (T.phy(r.ib) & (1u << (2 * a + side)))
         return ST_ESCAPED;
     const int ap0 = a == 2 ? 0 : a + 1;
-    const int dl = P.d_lvl[6 * r.ib + 2 * a + side];
+    const int dl = T.dl(r.ib, 2 * a + side);
     int j0 = 0, j1 = 0;
     if (dl > 0) {
 #pragma unroll
@@ -160,11 +161,11 @@ __device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, i
                 j1 = jj;
         }
     }
-    const int slot = 24 * r.ib + ((a * 2 + side) * 2 + j0) * 2 + j1;
-    const int nb = P.nb_id[slot];
+    const int slot = ((a * 2 + side) * 2 + j0) * 2 + j1;
+    const int nb = T.id(r.ib, slot);
     if (nb < 0)
         return ST_LOST;
-    const int shift = P.nb_shift[slot];
+    const int shift = T.shift(r.ib, slot);
     r.ib = nb;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -185,6 +186,12 @@ __device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, i
     reload = true;
     regulate(P, r, a, reload);
     return ST_IN;
+}
+
+__device__ __forceinline__ int change_block(const kparams &P, ray_t &r, int a, int side,
+                                            bool &reload)
+{
+    return change_block(P, r, a, side, reload, nb_global{P});
 }
 
 __device__ __forceinline__ int cross_face(const kparams &P, ray_t &r, int ap, double l,
