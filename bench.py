@@ -102,6 +102,10 @@ def parse():
     p.add_argument("--main-kind", default="tuned", choices=["tuned", "persistent", "block_queue"],
                    help="schedule kind of the main workload: the autotuner's pick screened at "
                         "full size, or that kind with the tuned knobs")
+    p.add_argument("--queue-knobs", default="",
+                   help="with --main-kind block_queue: passes,chunk,yield,blocks_per_cu of the "
+                        "queue instead of the tuned ones (for counter runs, under which the "
+                        "tuner's sample timings are slowed)")
     p.add_argument("--skip-adjoint", action="store_true")
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
@@ -970,8 +974,18 @@ def main():
         cands = [tuned, {**tuned, "event_batch": 8 if eb == 16 else 16}]
         if tuned["kind"] == _abi.KP_SCHED_BLOCKQ:
             cands[1] = {**tuned, "kind": _abi.KP_SCHED_PERSISTENT}
+            # and the queue's own default knobs, where the tuner moved off them
+            dflt = {**tuned, "blocks_per_cu": 4, "queue_passes": 0, "queue_chunk": 0,
+                    "queue_yield": 0}
+            if dflt != tuned:
+                cands.append(dflt)
         if main_kind == "block_queue":
-            cands = cands[:1]
+            cands = [c for c in cands if c["kind"] == _abi.KP_SCHED_BLOCKQ]
+            if args.queue_knobs:
+                qp, qc, qy, qb = (int(x) for x in args.queue_knobs.split(","))
+                cands = [{**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0,
+                          "queue_passes": qp, "queue_chunk": qc, "queue_yield": qy,
+                          "blocks_per_cu": qb}]
         check = []
         for c in cands:
             eng.set_schedule(**c)
@@ -984,7 +998,9 @@ def main():
             check.append(time.perf_counter() - t)
         eng.set_schedule(**cands[int(np.argmin(check))])
         eb_check["mixed" if prec == _abi.KP_F32 else "fp64"] = {
-            "candidates": [{"kind": c["kind"], "event_batch": c["event_batch"]} for c in cands],
+            "candidates": [{k: c[k] for k in ("kind", "event_batch", "blocks_per_cu",
+                                              "queue_passes", "queue_chunk", "queue_yield")}
+                           for c in cands],
             "ms": [1e3 * c for c in check]}
         for k in range(warmup):
             run.packet_offset = (1 << 50) + k * n

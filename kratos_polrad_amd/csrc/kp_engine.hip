@@ -1341,9 +1341,9 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         for (int b = 0; b < pl.bq_first_band; ++b)
             pl.bq_first_slot += run->bands[b].n_packets;
         pl.bq_n_slots = total - pl.bq_first_slot;
-        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 20;
-        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 1024u;
-        pl.bq_yield = S.queue_yield > 0 ? (uint32_t)S.queue_yield : 128u;
+        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 16;
+        pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 2048u;
+        pl.bq_yield = S.queue_yield > 0 ? (uint32_t)S.queue_yield : 64u;
         pl.bq_pass_blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
         pl.bq_finish_blocks = (uint64_t)e->n_cu * 4;
         if (pl.bq_n_slots > 0) {
@@ -2283,6 +2283,12 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                     kp_schedule c = base;
                     c.kind = KP_SCHED_BLOCKQ;
                     c.queue_from_band = k;
+                    // the queue's own defaults: all its pass workgroups
+                    // resident (a pass hands out tasks statically)
+                    c.blocks_per_cu = 4;
+                    c.queue_passes = 0;
+                    c.queue_chunk = 0;
+                    c.queue_yield = 0;
                     const float ms = time_it(c);
                     if (ms < queue_best) {
                         queue_best = ms;
@@ -2292,10 +2298,22 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                 if (queue_best < best_ms) {
                     best_ms = queue_best;
                     cur = queue_pick;
-                    try_knob(&kp_schedule::queue_chunk, {256, 2048});
-                    try_knob(&kp_schedule::queue_passes, {12, 16, 28});
-                    try_knob(&kp_schedule::queue_yield, {64, 256});
-                    try_knob(&kp_schedule::blocks_per_cu, {4, 8, 16});
+                    // passes and yield together: a packet's share of a pass
+                    // and the number of passes trade against each other
+                    const kp_schedule q0 = cur;
+                    for (const auto &py : {std::pair<int, int>{12, 64}, {20, 64}, {16, 128},
+                                           {24, 48}, {12, 128}}) {
+                        kp_schedule c = q0;
+                        c.queue_passes = py.first;
+                        c.queue_yield = py.second;
+                        const float ms = time_it(c);
+                        if (ms < best_ms) {
+                            best_ms = ms;
+                            cur = c;
+                        }
+                    }
+                    try_knob(&kp_schedule::queue_chunk, {1024, 4096});
+                    try_knob(&kp_schedule::blocks_per_cu, {8});
                 }
             }
             e->sched = cur;
