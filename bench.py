@@ -34,11 +34,11 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 #   persistent kernel: 563.08 GB fetched + 490.39 GB written = 1053.47 GB
 #   against 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md;
 #   a 5e6-packet dispatch gave 1.735);
-#   block queue, all dispatches of the step: 282.17 + 273.84 = 556.02 GB
+#   block queue, all dispatches of the step: 292.92 + 266.27 = 559.18 GB
 #   (profiles/block_queue_config2_amr3.md): below the algorithmic bytes,
 #   since the J tile takes the FP64 read-modify-writes of most crossings.
 TRAFFIC_PER_ALG_BYTE = {("config2_amr3_r2_lognormal_hg", 1): 1053.47 / 606.92,
-                        ("config2_amr3_r2_lognormal_hg", 2): 556.02 / 606.92}
+                        ("config2_amr3_r2_lognormal_hg", 2): 559.18 / 606.92}
 TRAFFIC_SOURCE = {("config2_amr3_r2_lognormal_hg", 1): (
     "rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic byte of the 1e8-packet "
     "k_persist dispatch, profiles/k_persist_pol32_config2_amr3.md"),

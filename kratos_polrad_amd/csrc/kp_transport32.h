@@ -187,12 +187,16 @@ __device__ __forceinline__ int cell_index(const kparams &P, const ray_t &r)
 }
 
 // cell_index with the indices clamped into the block: the cell itself while
-// the ray is in its block, a cell of the block after it stepped out.
+// the ray is in its block, some cell of the block after it stepped out (an
+// index of -1 maps to n-1: one unsigned minimum per axis).
 __device__ __forceinline__ int cell_index_clamped(const kparams &P, const ray_t &r)
 {
-    const int m = P.n_cell - 1;
-    return min(max(r.i[0], 0), m) +
-           P.n_cell * (min(max(r.i[1], 0), m) + P.n_cell * min(max(r.i[2], 0), m));
+    const uint32_t m = (uint32_t)P.n_cell - 1u;
+    // 24-bit products: cell indices of a block stay far below 2^24
+    return (int)(min((uint32_t)r.i[0], m) +
+                 __umul24((uint32_t)P.n_cell,
+                          min((uint32_t)r.i[1], m) +
+                              __umul24((uint32_t)P.n_cell, min((uint32_t)r.i[2], m))));
 }
 
 __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *dir, float ks,
@@ -241,7 +245,8 @@ __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const fl
     r.d[2] = dir[2];
     set_scales(P, r);
     double tau = 0.0;
-    float kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
+    const float *kb = P.kappa + (size_t)r.ib * P.n3; // the block's kappa_rho
+    float kc = kb[cell_index(P, r)];
     for (;;) {
         int ap, st;
         for (;;) {
@@ -257,7 +262,7 @@ __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const fl
             // value is not used), so kc is one register loaded one step
             // ahead: a conditional load here made the compiler copy it into
             // place, waiting for the load within the step
-            kc = P.kappa[(size_t)r.ib * P.n3 + cell_index_clamped(P, r)];
+            kc = kb[(uint32_t)cell_index_clamped(P, r)];
             tau += (double)(kap * l);
             ncross += 1;
             if (tau > kTauCut) {
@@ -271,7 +276,8 @@ __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const fl
             break;
         if (change_block(P, r, ap, st == ST_BLOCK_HI) != ST_IN)
             break;
-        kc = P.kappa[(size_t)r.ib * P.n3 + cell_index(P, r)];
+        kb = P.kappa + (size_t)r.ib * P.n3;
+        kc = kb[cell_index(P, r)];
     }
     return tau;
 }
