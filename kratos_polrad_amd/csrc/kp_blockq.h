@@ -351,6 +351,93 @@ template <class Pol> using tile_t = typename Pol::real;
 // wave and march instead of in nearly every step.
 template <class Pol> constexpr bool kQueueDeferBlock = true;
 
+// march<true> written with one exit from its step loop: each step ends in an
+// event code (none, yield, step budget, lost, interaction point, block edge)
+// and the loop runs while the code is none.  The steps and their order are
+// those of march; with one exit the compiler keeps each piece of lane state
+// in one register, where the five exits of march's step loop made it copy
+// the state at every step (41 of 111 VALU instructions of the mixed step).
+enum : int { EV_NONE = 0, EV_YIELD, EV_STEPS, EV_LOST, EV_HIT, EV_BLOCK };
+
+template <class Pol>
+__device__ int march_block(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
+                           tile_t<Pol> *lj)
+{
+    using real = typename Pol::real;
+    uint32_t work = 0; // steps and interactions in this pass
+    for (;;) {
+        int ev = EV_NONE;
+        int cell = 0;
+        do {
+            if (work >= Q.yield) {
+                ev = EV_YIELD;
+            } else {
+                ++work;
+                if (L.steps <= 0) {
+                    ev = EV_STEPS;
+                } else {
+                    real l;
+                    const int ap = Pol::dl(L.r, l);
+                    if (ap < 0) {
+                        ev = EV_LOST;
+                    } else {
+                        cell = Pol::cell(P, L.r);
+                        const real kap = (real)lk[cell] * L.ks;
+                        const real dtau = kap * l;
+                        L.ncross += 1;
+                        L.steps -= 1;
+                        bool hit;
+                        real seg;
+                        if constexpr (sizeof(real) == 8) {
+                            hit = kap > 0.0 && dtau >= L.tau;
+                            seg = l;
+                            if (hit) {
+                                const double li = L.tau / kap;
+                                seg = li > l ? l : li;
+                            }
+                        } else {
+                            hit = kap > 0.0f && (double)dtau >= L.tau;
+                            seg = hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
+                        }
+                        if (P.J) {
+                            real v;
+                            if constexpr (sizeof(real) == 8)
+                                v = L.I * (seg / P.vol0[L.r.ib]);
+                            else
+                                v = L.I * seg * L.r.iv;
+                            atomicAdd(&lj[cell], v);
+                        }
+                        if (hit) {
+                            Pol::advance(L.r, seg);
+                            ev = EV_HIT;
+                        } else {
+                            L.tau = L.tau - (double)dtau;
+                            const int st = Pol::cross_step(P, L.r, ap, l);
+                            if (st != f32::ST_IN) {
+                                L.blk = ap + (st == f32::ST_BLOCK_HI ? 4 : 0);
+                                ev = EV_BLOCK;
+                            }
+                        }
+                    }
+                }
+            }
+        } while (ev == EV_NONE);
+        if (ev == EV_YIELD)
+            return MARCH_OUT;
+        if (ev == EV_BLOCK)
+            return MARCH_BLOCK;
+        if (ev != EV_HIT) {
+            L.status = (ev == EV_STEPS) ? KP_END_STEPS : KP_END_LOST;
+            return MARCH_END;
+        }
+        if (!interact<Pol>(P, Q, L, (int64_t)L.r.ib * P.n3 + cell)) {
+            L.status = KP_END_RR;
+            return MARCH_END;
+        }
+        ++work;
+    }
+}
+
 template <class Pol, bool LOCAL>
 __device__ int march(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
                      tile_t<Pol> *lj)

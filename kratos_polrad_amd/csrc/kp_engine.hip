@@ -572,7 +572,7 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
                 for (uint64_t q = p + threadIdx.x; q < e; q += blockDim.x) {
                     kp::bq::qlane<Pol> L;
                     kp::bq::load<Pol>(P, in[Q.order[q]], L);
-                    int m = kp::bq::march<Pol, true>(P, Q, L, lk, lj);
+                    int m = kp::bq::march_block<Pol>(P, Q, L, lk, lj);
                     if (m == kp::bq::MARCH_BLOCK)
                         m = kp::bq::leave_block<Pol>(P, L, kp::nb_global{P});
                     if (m == kp::bq::MARCH_OUT)
