@@ -323,7 +323,9 @@ def batched(steps, scale, bands="0:64", kind="screen", fixed=False):
             for k in sorted({0, first_thin}):
                 if k < len(w.run.bands):
                     cands[f"block_queue_from_{k}"] = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ,
-                                                      "queue_from_band": k}
+                                                      "queue_from_band": k, "blocks_per_cu": 4,
+                                                      "queue_passes": 0, "queue_chunk": 0,
+                                                      "queue_yield": 0}
         screen = {}
         for name, sc in cands.items():
             screen[name] = 1e3 * run_graph(prec, sc, 2)[0] / 2
@@ -579,8 +581,13 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
         best = None
         cands = [{**tuned, "coherent": 0}, {**tuned, "coherent": 1}]
         if tuned["kind"] != _abi.KP_SCHED_BLOCKQ:
-            cands.append({**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0,
-                          "coherent": 0})
+            # the queue with its own default knobs (every pass workgroup
+            # resident), and with twice its default passes: the disk's packets
+            # stay alive for many passes, and a 4e6-packet sample
+            # undervalues passes (their fixed cost is a larger share of it)
+            qd = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0, "coherent": 0,
+                  "blocks_per_cu": 4, "queue_passes": 0, "queue_chunk": 0, "queue_yield": 0}
+            cands += [qd, {**qd, "queue_passes": 64}]
         for sc in cands:
             eng.set_schedule(**sc)
             o = eng.alloc_outputs(run)
@@ -595,7 +602,7 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
                 scheds[prec] = sc
             del o
             progress(f"large: one chunk, precision {prec}, kind {sc['kind']}, "
-                     f"order {sc['coherent']}, {dt:.1f} s")
+                     f"order {sc['coherent']}, passes {sc['queue_passes']}, {dt:.2f} s")
         runs[prec] = best
     res["fp64_one_chunk"] = {"seconds": runs[_abi.KP_F64][0], "value": runs[_abi.KP_F64][1],
                              "schedule": scheds[_abi.KP_F64]}

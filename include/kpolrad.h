@@ -321,7 +321,7 @@ typedef struct kp_schedule {
     int32_t adjoint_chunk;
     /* KP_SCHED_BLOCKQ: first band of the block queue (0 = every band; the
      * bands before it run on the persistent kernel), passes before the
-     * packets still alive finish one thread each (0 = 16), sorted packets
+     * packets still alive finish one thread each (0 = 32), sorted packets
      * per workgroup task (0 = 2048), and the march steps plus interactions a
      * packet may take in one pass before it goes back to the queue in its
      * block (0 = 64), which bounds the slowest lane of a pass. */

@@ -80,6 +80,8 @@ struct qparams {
     const uint32_t *in_key;           // their keys
     const int32_t *order;             // sorted position -> record
     const uint32_t *seg;              // [n_keys + 1] first sorted position of each key
+    const uint2 *tasks;               // (first sorted position, key) of each task
+    const unsigned long long *n_tasks; // tasks of this pass
     const unsigned long long *n_in;   // records of this pass
     void *out;                        // records that crossed into another block
     uint32_t *out_key;

@@ -52,9 +52,13 @@ struct fwd_bufs {
     uint32_t *bq_key[2] = {nullptr, nullptr};
     int32_t *bq_order = nullptr;
     uint32_t *bq_skey = nullptr, *bq_seg = nullptr, *bq_np = nullptr;
+    uint32_t *bq_toff = nullptr; // [n_keys + 1] first task of each key
+    uint2 *bq_task = nullptr;    // tasks of a pass: (first sorted position, key)
     void *bq_ws = nullptr;
-    unsigned long long *bq_cnt = nullptr; // [2] records, [2] queued peel-off rays
+    // [0], [1] records, [2] queued peel-off rays, [3] tasks of the pass
+    unsigned long long *bq_cnt = nullptr;
     size_t bq_cap = 0, bq_rec_bytes = 0, bq_seg_cap = 0, bq_np_cap = 0, bq_ws_cap = 0;
+    size_t bq_task_cap = 0;
     void *bq_rays = nullptr;
     size_t bq_rays_bytes = 0;
 };
@@ -517,6 +521,71 @@ __global__ void __launch_bounds__(256) k_bq_seg(const uint32_t *skey,
     }
 }
 
+// Tasks of a pass: each key's run of sorted positions split into pieces of at
+// most `chunk`, so a task lies in one block (one staging of its kappa_rho and
+// J tile) and the runs of sparse late passes spread over the workgroups
+// instead of one workgroup walking many blocks.  toff[k] = number of tasks of
+// key k (k_bq_tcount), then its exclusive prefix sum (k_bq_tscan, one
+// workgroup; toff[n_keys] and *n_tasks the total), then one thread per task
+// finds its key by binary search over toff (k_bq_tfill).
+__global__ void __launch_bounds__(256) k_bq_tcount(const uint32_t *seg, uint32_t n_keys,
+                                                   uint32_t chunk, uint32_t *toff)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_keys; k += stride)
+        toff[k] = (seg[k + 1] - seg[k] + chunk - 1) / chunk;
+}
+
+__global__ void __launch_bounds__(1024) k_bq_tscan(uint32_t *toff, uint32_t n_keys,
+                                                   unsigned long long *n_tasks)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (n_keys + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = min(b + per, n_keys);
+    uint32_t sum = 0;
+    for (uint32_t k = b; k < e; ++k)
+        sum += toff[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    // inclusive scan of the 1024 partial sums (Hillis-Steele)
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum; // exclusive
+    for (uint32_t k = b; k < e; ++k) {
+        const uint32_t c = toff[k];
+        toff[k] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) {
+        toff[n_keys] = part[1023];
+        *n_tasks = part[1023];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bq_tfill(const uint32_t *seg, const uint32_t *toff,
+                                                  uint32_t n_keys, uint32_t chunk,
+                                                  const unsigned long long *n_tasks,
+                                                  uint2 *task)
+{
+    const uint64_t nt = *n_tasks;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        uint32_t lo = 0, hi = n_keys - 1; // last key with toff[k] <= t
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (toff[mid] <= (uint32_t)t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        task[t] = make_uint2(seg[lo] + ((uint32_t)t - toff[lo]) * chunk, lo);
+    }
+}
+
 // One pass of the block queue: tasks of `chunk` sorted records; per block of
 // a task the block's kappa_rho and a J tile in LDS, each packet marched to
 // the block's edge or its end (in the FP64 path by the wave loop pass_block,
@@ -530,7 +599,7 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     tile *lj = reinterpret_cast<tile *>(kp::kp_lds);
     float *lk = reinterpret_cast<float *>(lj + P.n3);
     __shared__ uint32_t s_key;
-    __shared__ uint64_t s_end;
+    __shared__ uint64_t s_p, s_end;
     __shared__ uint32_t s_next;
     __shared__ kp::blk_nb_t s_nb;
     __shared__ unsigned long long s_cnt[KP_NCOUNTERS];
@@ -538,23 +607,24 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     const uint64_t n = *Q.n_in;
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
     kp::bq::wave_peel_open();
-    for (uint64_t t = blockIdx.x; t * Q.chunk < n; t += gridDim.x) {
-        uint64_t p = t * Q.chunk;
-        const uint64_t p1 = (p + Q.chunk < n) ? p + Q.chunk : n;
-        while (p < p1) {
+    const uint64_t n_tasks = *Q.n_tasks;
+    for (uint64_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+        {
             if (threadIdx.x == 0) {
-                const uint32_t key = Q.in_key[Q.order[p]];
-                uint64_t e = Q.seg[key + 1];
-                if (e > p1)
-                    e = p1;
-                if (e <= p) // progress whatever the table holds
-                    e = p1;
-                s_key = key;
+                const uint2 tk = Q.tasks[t];
+                uint64_t e = Q.seg[tk.y + 1];
+                if (e > (uint64_t)tk.x + Q.chunk)
+                    e = (uint64_t)tk.x + Q.chunk;
+                if (e > n)
+                    e = n;
+                s_key = tk.y;
+                s_p = tk.x;
                 s_end = e;
                 s_next = 0;
             }
             __syncthreads();
             const uint32_t key = s_key;
+            const uint64_t p = s_p;
             const uint64_t e = s_end;
             const int ib = (int)(key % (uint32_t)Q.n_blocks);
             const int band = Q.first_band + (int)(key / (uint32_t)Q.n_blocks);
@@ -591,7 +661,6 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
                 }
             }
             __syncthreads();
-            p = e;
         }
     }
     kp::bq::wave_peel_close<real>(Q);
@@ -941,11 +1010,14 @@ void free_bufs(fwd_bufs &b)
     dfree(b.bq_order);
     dfree(b.bq_skey);
     dfree(b.bq_seg);
+    dfree(b.bq_toff);
+    dfree(b.bq_task);
     dfree(b.bq_np);
     dfree(b.bq_ws);
     dfree(b.bq_cnt);
     dfree(b.bq_rays);
     b.bq_cap = b.bq_rec_bytes = b.bq_seg_cap = b.bq_np_cap = b.bq_ws_cap = 0;
+    b.bq_task_cap = 0;
     b.bq_rays_bytes = 0;
 }
 
@@ -1235,7 +1307,8 @@ struct fwd_plan {
 };
 
 // Block-queue buffers of a plan, grown to n records of rec_bytes each.
-void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int n_bands)
+void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int n_bands,
+               uint32_t chunk)
 {
     if (fb.bq_cap < n || fb.bq_rec_bytes < rec_bytes) {
         for (int k = 0; k < 2; ++k) {
@@ -1256,9 +1329,20 @@ void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int 
     }
     if (fb.bq_seg_cap < (size_t)n_keys + 1) {
         dfree(fb.bq_seg);
+        dfree(fb.bq_toff);
         fb.bq_seg_cap = 0;
         fb.bq_seg = dmalloc<uint32_t>((size_t)n_keys + 1);
+        fb.bq_toff = dmalloc<uint32_t>((size_t)n_keys + 1);
         fb.bq_seg_cap = (size_t)n_keys + 1;
+    }
+    // a key of len records makes ceil(len / chunk) tasks: at most n / chunk
+    // plus one per key
+    const size_t nt = (size_t)(n / chunk) + n_keys + 1;
+    if (fb.bq_task_cap < nt) {
+        dfree(fb.bq_task);
+        fb.bq_task_cap = 0;
+        fb.bq_task = dmalloc<uint2>(nt);
+        fb.bq_task_cap = nt;
     }
     if (fb.bq_np_cap < (size_t)n_bands) {
         dfree(fb.bq_np);
@@ -1274,7 +1358,7 @@ void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int 
         fb.bq_ws_cap = wsb;
     }
     if (!fb.bq_cnt)
-        fb.bq_cnt = dmalloc<unsigned long long>(3);
+        fb.bq_cnt = dmalloc<unsigned long long>(4);
 }
 
 // Peel-off ray queue of the block queue: `rays` records of ray_bytes each.
@@ -1341,7 +1425,7 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
         for (int b = 0; b < pl.bq_first_band; ++b)
             pl.bq_first_slot += run->bands[b].n_packets;
         pl.bq_n_slots = total - pl.bq_first_slot;
-        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 16;
+        pl.bq_passes = S.queue_passes > 0 ? S.queue_passes : 32;
         pl.bq_chunk = S.queue_chunk > 0 ? (uint32_t)S.queue_chunk : 2048u;
         pl.bq_yield = S.queue_yield > 0 ? (uint32_t)S.queue_yield : 64u;
         pl.bq_pass_blocks = (uint64_t)e->n_cu * S.blocks_per_cu;
@@ -1360,7 +1444,7 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
                 ++pl.bq_key_bits;
             const size_t rb = run->precision == KP_F64 ? sizeof(kp::bq::rec_t<double>)
                                                        : sizeof(kp::bq::rec_t<float>);
-            ensure_bq(fb, pl.bq_n_slots, rb, pl.bq_n_keys, run->n_bands);
+            ensure_bq(fb, pl.bq_n_slots, rb, pl.bq_n_keys, run->n_bands, pl.bq_chunk);
             // peel-off records of a pass (one per interaction, all observers):
             // two per packet, at least 2^20 and at most 2^27 records or 8 GB;
             // interactions beyond it march their peel-offs in place
@@ -1466,11 +1550,20 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
                            fb.bq_key[cur], fb.bq_order, fb.bq_cnt + cur, fb.bq_skey);
         hipLaunchKernelGGL(k_bq_seg, dim3(grid_n((uint64_t)pl.bq_n_keys + 1)), dim3(256), 0,
                            stream, fb.bq_skey, fb.bq_cnt + cur, pl.bq_n_keys, fb.bq_seg);
+        hipLaunchKernelGGL(k_bq_tcount, dim3(grid_n(pl.bq_n_keys)), dim3(256), 0, stream,
+                           fb.bq_seg, pl.bq_n_keys, pl.bq_chunk, fb.bq_toff);
+        hipLaunchKernelGGL(k_bq_tscan, dim3(1), dim3(1024), 0, stream, fb.bq_toff,
+                           pl.bq_n_keys, fb.bq_cnt + 3);
+        hipLaunchKernelGGL(k_bq_tfill, dim3(grid_n(fb.bq_task_cap)), dim3(256), 0, stream,
+                           fb.bq_seg, fb.bq_toff, pl.bq_n_keys, pl.bq_chunk, fb.bq_cnt + 3,
+                           fb.bq_task);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.bq_cnt + nxt);
         Q.in = fb.bq_rec[cur];
         Q.in_key = fb.bq_key[cur];
         Q.order = fb.bq_order;
         Q.seg = fb.bq_seg;
+        Q.tasks = fb.bq_task;
+        Q.n_tasks = fb.bq_cnt + 3;
         Q.n_in = fb.bq_cnt + cur;
         Q.out = fb.bq_rec[nxt];
         Q.out_key = fb.bq_key[nxt];
@@ -2301,8 +2394,8 @@ kp_status kp_autotune(kp_engine *e, const kp_source *src, const kp_run *run,
                     // passes and yield together: a packet's share of a pass
                     // and the number of passes trade against each other
                     const kp_schedule q0 = cur;
-                    for (const auto &py : {std::pair<int, int>{12, 64}, {20, 64}, {16, 128},
-                                           {24, 48}, {12, 128}}) {
+                    for (const auto &py : {std::pair<int, int>{20, 64}, {48, 64}, {64, 64},
+                                           {96, 64}, {32, 128}, {64, 128}}) {
                         kp_schedule c = q0;
                         c.queue_passes = py.first;
                         c.queue_yield = py.second;
