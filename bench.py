@@ -646,7 +646,9 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
     t0 = time.perf_counter()
     gr = eng.adjoint(w.source, w.run, wgt)
     torch.cuda.synchronize()
-    res["gradient_pass"] = {"seconds": time.perf_counter() - t0, "packets": chunk,
+    tg = time.perf_counter() - t0
+    res["gradient_pass"] = {"seconds": tg, "packets": chunk,
+                            "over_forward_chunk": tg / runs[_abi.KP_F32][0],
                             "d_logrho_norm": float(torch.linalg.vector_norm(gr["d_logrho"])),
                             "jlin": float(gr["jlin"][0])}
     del gr
