@@ -34,11 +34,11 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 #   persistent kernel: 563.08 GB fetched + 490.39 GB written = 1053.47 GB
 #   against 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md;
 #   a 5e6-packet dispatch gave 1.735);
-#   block queue, all dispatches of the step: 292.92 + 266.27 = 559.18 GB
+#   block queue, all dispatches of the step: 297.35 + 253.37 = 550.72 GB
 #   (profiles/block_queue_config2_amr3.md): below the algorithmic bytes,
 #   since the J tile takes the FP64 read-modify-writes of most crossings.
 TRAFFIC_PER_ALG_BYTE = {("config2_amr3_r2_lognormal_hg", 1): 1053.47 / 606.92,
-                        ("config2_amr3_r2_lognormal_hg", 2): 559.18 / 606.92}
+                        ("config2_amr3_r2_lognormal_hg", 2): 550.72 / 606.92}
 TRAFFIC_SOURCE = {("config2_amr3_r2_lognormal_hg", 1): (
     "rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic byte of the 1e8-packet "
     "k_persist dispatch, profiles/k_persist_pol32_config2_amr3.md"),
@@ -112,6 +112,9 @@ def parse():
     p.add_argument("--skip-compaction", action="store_true")
     p.add_argument("--skip-large", action="store_true")
     p.add_argument("--large-packets", type=float, default=1e9)
+    p.add_argument("--large-passes", default="0,64",
+                   help="block-queue pass counts screened on one chunk of the large workload "
+                        "(0 = the queue's default)")
     p.add_argument("--fd-cells", type=int, default=64,
                    help="cells of the per-cell finite-difference check of the adjoint (0: off)")
     p.add_argument("--fd-packets", type=float, default=1e7,
@@ -550,7 +553,7 @@ def adjoint_fd(n=10_000_000, n_cells=64, eps=1e-3, n_batches=8, seed=3, eps_dir=
                             "max_rel_diff": float(d_rel.max())}}
 
 
-def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
+def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0, queue_passes=(0, 64)):
     """Config 5: 1e9 packets on the ~28,000-block disk mesh, streamed as device
     chunks of `chunk` packets replayed from one hipGraph (autotuned schedule,
     mixed precision, packet order re-checked on one chunk); FP64 on one chunk
@@ -587,7 +590,7 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0):
             # undervalues passes (their fixed cost is a larger share of it)
             qd = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0, "coherent": 0,
                   "blocks_per_cu": 4, "queue_passes": 0, "queue_chunk": 0, "queue_yield": 0}
-            cands += [qd, {**qd, "queue_passes": 64}]
+            cands += [{**qd, "queue_passes": qp} for qp in queue_passes]
         for sc in cands:
             eng.set_schedule(**sc)
             o = eng.alloc_outputs(run)
@@ -1111,7 +1114,8 @@ def main():
 
     if not args.skip_large:
         res["large"] = large(int(args.large_packets), min(int(args.large_packets), 100_000_000),
-                             0.0 if args.no_cpu else args.cpu_seconds / 2)
+                             0.0 if args.no_cpu else args.cpu_seconds / 2,
+                             tuple(int(x) for x in args.large_passes.split(",") if x))
         progress("large run done")
 
     if not args.skip_sph:
