@@ -82,6 +82,7 @@ struct qparams {
     const uint32_t *seg;              // [n_keys + 1] first sorted position of each key
     const uint2 *tasks;               // (first sorted position, key) of each task
     const unsigned long long *n_tasks; // tasks of this pass
+    unsigned long long *task_next;     // next task to claim (zeroed with the task table)
     const unsigned long long *n_in;   // records of this pass
     void *out;                        // records that crossed into another block
     uint32_t *out_key;
@@ -97,6 +98,7 @@ struct qparams {
     // peel-off rays of this pass (null: every lane marches its own)
     void *pr;
     unsigned long long *pr_count;
+    unsigned long long *peel_next;    // next ray the peel kernel hands out (zeroed with pr_count)
     uint64_t pr_cap;
 };
 
@@ -361,11 +363,128 @@ template <class Pol> constexpr bool kQueueDeferBlock = true;
 // the state at every step (41 of 111 VALU instructions of the mixed step).
 enum : int { EV_NONE = 0, EV_YIELD, EV_STEPS, EV_LOST, EV_HIT, EV_BLOCK };
 
+// The mixed path's march_block.  Within its step loop the ray keeps,
+// instead of the integer cell indices, the face ahead on each axis as a
+// float (the value f32::get_dl converts the index to) and the linear cell
+// index, moved by +-1, +-n or +-n^2; the indices are rebuilt from the faces
+// when the loop ends.  Lengths, positions and optical depths are computed as
+// in get_dl and cross_in_block, so they are the same numbers.
+template <class Pol>
+__device__ int march_block_f32(const kparams &P, const qparams &Q, qlane<Pol> &L,
+                               const float *lk, tile_t<Pol> *lj)
+{
+    static_assert(sizeof(typename Pol::real) == 4, "mixed path only");
+    const int n = P.n_cell;
+    const float nf = (float)n;
+    uint32_t work = 0; // steps and interactions in this pass
+    for (;;) {
+        // per direction (set again after each scattering)
+        const bool nz0 = L.r.d[0] != 0.0f, nz1 = L.r.d[1] != 0.0f, nz2 = L.r.d[2] != 0.0f;
+        const f32::face_steps S(L.r.d, n);
+        float ff[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            ff[a] = (float)(L.r.i[a] + S.up(a));
+        int c = f32::cell_index(P, L.r);
+        int ev = EV_NONE;
+        int cell = 0;
+        do {
+            if (work >= Q.yield) {
+                ev = EV_YIELD;
+            } else {
+                ++work;
+                if (L.steps <= 0) {
+                    ev = EV_STEPS;
+                } else {
+                    // get_dl: the nearest face ahead, ties to the later axis
+                    float t[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        t[a] = fmaxf((ff[a] - L.r.x[a]) * L.r.lpc[a], 0.0f);
+                    int ap = -1;
+                    float l = f32::kInf;
+                    if (t[0] <= l && nz0) {
+                        ap = 0;
+                        l = t[0];
+                    }
+                    if (t[1] <= l && nz1) {
+                        ap = 1;
+                        l = t[1];
+                    }
+                    if (t[2] <= l && nz2) {
+                        ap = 2;
+                        l = t[2];
+                    }
+                    if (ap < 0) {
+                        ev = EV_LOST;
+                    } else {
+                        cell = c;
+                        const float kap = lk[cell] * L.ks;
+                        const float dtau = kap * l;
+                        L.ncross += 1;
+                        L.steps -= 1;
+                        const bool hit = kap > 0.0f && (double)dtau >= L.tau;
+                        const float seg =
+                            hit ? fminf((float)L.tau * f32::fast_rcp(kap), l) : l;
+                        if (P.J)
+                            atomicAdd(&lj[cell], L.I * seg * L.r.iv);
+                        if (hit) {
+                            Pol::advance(L.r, seg);
+                            ev = EV_HIT;
+                        } else {
+                            L.tau = L.tau - (double)dtau;
+                            // cross_in_block: the exit axis onto its face, the
+                            // others along the ray
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                if (a == ap) {
+                                    L.r.x[a] = ff[a];
+                                    ff[a] += S.sf(a);
+                                    c += S.sc(a);
+                                } else {
+                                    L.r.x[a] = fmaf(l, L.r.cpl[a], L.r.x[a]);
+                                }
+                            }
+                            const float fhi = fmaxf(fmaxf(ff[0], ff[1]), ff[2]);
+                            const float flo = fminf(fminf(ff[0], ff[1]), ff[2]);
+                            if (fhi > nf) {
+                                L.blk = ap + 4;
+                                ev = EV_BLOCK;
+                            } else if (flo < 0.0f) {
+                                L.blk = ap;
+                                ev = EV_BLOCK;
+                            }
+                        }
+                    }
+                }
+            }
+        } while (ev == EV_NONE);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            L.r.i[a] = (int)ff[a] - S.up(a);
+        if (ev == EV_YIELD)
+            return MARCH_OUT;
+        if (ev == EV_BLOCK)
+            return MARCH_BLOCK;
+        if (ev != EV_HIT) {
+            L.status = (ev == EV_STEPS) ? KP_END_STEPS : KP_END_LOST;
+            return MARCH_END;
+        }
+        if (!interact<Pol>(P, Q, L, (int64_t)L.r.ib * P.n3 + cell)) {
+            L.status = KP_END_RR;
+            return MARCH_END;
+        }
+        ++work;
+    }
+}
+
 template <class Pol>
 __device__ int march_block(const kparams &P, const qparams &Q, qlane<Pol> &L, const float *lk,
                            tile_t<Pol> *lj)
 {
     using real = typename Pol::real;
+    if constexpr (sizeof(real) == 4)
+        return march_block_f32<Pol>(P, Q, L, lk, lj);
     uint32_t work = 0; // steps and interactions in this pass
     for (;;) {
         int ev = EV_NONE;

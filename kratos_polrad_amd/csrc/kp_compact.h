@@ -18,4 +18,11 @@ void sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void 
                 size_t ws_bytes, hipStream_t stream,
                 const unsigned long long *n_dev = nullptr);
 
+// sort_order without the copy of the permutation: *sorted_idx and
+// *sorted_keys point into ws, at the permutation and at the keys in sorted
+// order, valid until ws is used again.
+void sort_in_place(const int32_t *keys, uint64_t n, int bits, void *ws, size_t ws_bytes,
+                   hipStream_t stream, const unsigned long long *n_dev,
+                   const int32_t **sorted_idx, const uint32_t **sorted_keys);
+
 } // namespace kp

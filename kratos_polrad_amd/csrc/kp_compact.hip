@@ -562,15 +562,18 @@ void *static_scratch()
 
 } // namespace
 
-void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void *ws,
-                    size_t ws_bytes, hipStream_t s, const unsigned long long *n_dev)
+namespace {
+
+// The digit passes of sort_order; returns the buffers holding the sorted
+// keys and the permutation (inside ws).
+void sort_passes(const int32_t *keys, uint64_t n, int bits, void *ws, size_t ws_bytes,
+                 hipStream_t s, const unsigned long long *n_dev, const uint32_t **kin_out,
+                 const uint32_t **iin_out)
 {
     if (n >= (1ull << 31) || bits < 1 || bits > 30)
         throw kp::error(KP_ERR_ARG, "sort_order: n must be below 2^31 and bits in [1,30]");
-    if (n == 0)
-        return;
     const bin_layout L = bin_ws(n);
-    if (!keys || !order || !ws || ws_bytes < L.total)
+    if (!keys || !ws || ws_bytes < L.total)
         throw kp::error(KP_ERR_ARG, "sort_order: bad buffers");
     char *w = (char *)ws;
     uint32_t *ka = (uint32_t *)(w + L.ka), *kb = (uint32_t *)(w + L.kb);
@@ -602,9 +605,37 @@ void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, v
         kout = (kout == ka) ? kb : ka;
         iout = (iout == ia) ? ib : ia;
     }
+    KP_HIPC(hipGetLastError());
+    *kin_out = kin;
+    *iin_out = iin;
+}
+
+} // namespace
+
+void kp::sort_order(const int32_t *keys, uint64_t n, int bits, int32_t *order, void *ws,
+                    size_t ws_bytes, hipStream_t s, const unsigned long long *n_dev)
+{
+    if (n == 0)
+        return;
+    if (!order)
+        throw kp::error(KP_ERR_ARG, "sort_order: bad buffers");
+    const uint32_t *kin = nullptr, *iin = nullptr;
+    sort_passes(keys, n, bits, ws, ws_bytes, s, n_dev, &kin, &iin);
     hipLaunchKernelGGL(k_copy_order, dim3(grid_for(n)), dim3(kThreads), 0, s, iin, n, order,
                        n_dev);
     KP_HIPC(hipGetLastError());
+}
+
+void kp::sort_in_place(const int32_t *keys, uint64_t n, int bits, void *ws, size_t ws_bytes,
+                       hipStream_t s, const unsigned long long *n_dev,
+                       const int32_t **sorted_idx, const uint32_t **sorted_keys)
+{
+    if (n == 0 || !sorted_idx || !sorted_keys)
+        throw kp::error(KP_ERR_ARG, "sort_in_place: empty sort or null output");
+    const uint32_t *kin = nullptr, *iin = nullptr;
+    sort_passes(keys, n, bits, ws, ws_bytes, s, n_dev, &kin, &iin);
+    *sorted_idx = reinterpret_cast<const int32_t *>(iin);
+    *sorted_keys = kin;
 }
 
 extern "C" {

@@ -118,6 +118,9 @@ struct kparams {
     // replica b % R, spreading FP64 atomics on hot cells/pixels
     uint32_t j_nrep, img_nrep;
     uint64_t j_stride, img_stride; // elements per replica
+    // 0: a pixel's I, Q, U side by side (the caller's layout); otherwise the
+    // replicas hold three planes of img_plane pixel entries each
+    uint64_t img_plane;
     uint64_t j_cs, j_bs;           // J element of (cell gc, band b): gc*j_cs + b*j_bs
     // adjoint (path replay) launches
     const double *adj_w;           // dL/dimage, image layout
@@ -199,6 +202,16 @@ __device__ __forceinline__ double *j_at(const kparams &P, uint64_t gc, int b)
 __device__ __forceinline__ double *img_base(const kparams &P)
 {
     return P.image + (uint64_t)(blockIdx.x % P.img_nrep) * P.img_stride;
+}
+
+// Stokes component k (0 I, 1 Q, 2 U) of pixel entry px (band, observer and
+// pixel as pixel() numbers them) in the workgroup's image.  In the planar
+// replicas the three atomic adds of a peel-off go to three cache lines; side
+// by side in one line, each waited in the L2 for the one before.
+__device__ __forceinline__ double *img_at(const kparams &P, long px, int k)
+{
+    double *b = img_base(P);
+    return P.img_plane ? b + (uint64_t)k * P.img_plane + (uint64_t)px : b + 3 * px + k;
 }
 
 __device__ __forceinline__ void atomic_add_f64(double *p, double v)

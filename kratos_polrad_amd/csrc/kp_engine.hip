@@ -50,12 +50,12 @@ struct fwd_bufs {
     // emission peel-off crossings per band
     void *bq_rec[2] = {nullptr, nullptr};
     uint32_t *bq_key[2] = {nullptr, nullptr};
-    int32_t *bq_order = nullptr;
-    uint32_t *bq_skey = nullptr, *bq_seg = nullptr, *bq_np = nullptr;
+    uint32_t *bq_seg = nullptr, *bq_np = nullptr;
     uint32_t *bq_toff = nullptr; // [n_keys + 1] first task of each key
     uint2 *bq_task = nullptr;    // tasks of a pass: (first sorted position, key)
     void *bq_ws = nullptr;
-    // [0], [1] records, [2] queued peel-off rays, [3] tasks of the pass
+    // [0], [1] records, [2] queued peel-off rays, [3] tasks of the pass,
+    // [4] next task to claim
     unsigned long long *bq_cnt = nullptr;
     size_t bq_cap = 0, bq_rec_bytes = 0, bq_seg_cap = 0, bq_np_cap = 0, bq_ws_cap = 0;
     size_t bq_task_cap = 0;
@@ -224,6 +224,11 @@ __global__ void __launch_bounds__(256) k_forward_f32(kp::kparams P)
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_zero_u64x2(unsigned long long *p, unsigned long long *q)
+{
+    *p = 0ull;
+    *q = 0ull;
+}
 __global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
@@ -238,6 +243,29 @@ __global__ void __launch_bounds__(256) k_fold(double *out, double *rep, uint64_t
             rep[r * n + e] = 0.0;
         }
         out[e] += s;
+    }
+}
+
+// Planar image replicas (I, Q and U planes of np pixel entries each)
+// folded into the caller's image, a pixel's I, Q, U side by side:
+// out[3p + k] += sum_r rep[r][k][p]; the replicas are left zero.
+__global__ void __launch_bounds__(256) k_fold_planar(double *out, double *rep, uint64_t np,
+                                                     uint32_t R)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (uint32_t r = 0; r < R; ++r) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double *q = rep + ((uint64_t)r * 3 + k) * np + p;
+                s[k] += *q;
+                *q = 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            out[3 * p + k] += s[k];
     }
 }
 
@@ -450,7 +478,7 @@ __global__ void __launch_bounds__(64) k_bq_emit_peel(kp::kparams P, kp::bq::qpar
                 px = kp::f32::pixel(P, b, o, gx);
             }
             if (P.image && px >= 0)
-                kp::atomic_add_f64(kp::img_base(P) + 3 * px, (double)n * v);
+                kp::atomic_add_f64(kp::img_at(P, px, 0), (double)n * v);
         }
     }
     Q.emit_np[b] = (uint32_t)np;
@@ -491,16 +519,6 @@ __global__ void __launch_bounds__(256) k_bq_generate(kp::kparams P, kp::bq::qpar
         *Q.n_out = Q.n_slots;
 }
 
-// skey[p] = key of the p-th sorted record
-__global__ void __launch_bounds__(256) k_bq_skey(const uint32_t *key, const int32_t *order,
-                                                 const unsigned long long *n_p, uint32_t *skey)
-{
-    const uint64_t n = *n_p;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
-        skey[p] = key[order[p]];
-}
-
 // seg[k] = first sorted position whose key is >= k, for k in [0, n_keys]
 __global__ void __launch_bounds__(256) k_bq_seg(const uint32_t *skey,
                                                 const unsigned long long *n_p, uint32_t n_keys,
@@ -537,7 +555,8 @@ __global__ void __launch_bounds__(256) k_bq_tcount(const uint32_t *seg, uint32_t
 }
 
 __global__ void __launch_bounds__(1024) k_bq_tscan(uint32_t *toff, uint32_t n_keys,
-                                                   unsigned long long *n_tasks)
+                                                   unsigned long long *n_tasks,
+                                                   unsigned long long *task_next)
 {
     __shared__ uint32_t part[1024];
     const uint32_t per = (n_keys + 1023) / 1024;
@@ -563,6 +582,7 @@ __global__ void __launch_bounds__(1024) k_bq_tscan(uint32_t *toff, uint32_t n_ke
     if (threadIdx.x == 1023) {
         toff[n_keys] = part[1023];
         *n_tasks = part[1023];
+        *task_next = 0;
     }
 }
 
@@ -599,7 +619,7 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     tile *lj = reinterpret_cast<tile *>(kp::kp_lds);
     float *lk = reinterpret_cast<float *>(lj + P.n3);
     __shared__ uint32_t s_key;
-    __shared__ uint64_t s_p, s_end;
+    __shared__ uint64_t s_t, s_p, s_end;
     __shared__ uint32_t s_next;
     __shared__ kp::blk_nb_t s_nb;
     __shared__ unsigned long long s_cnt[KP_NCOUNTERS];
@@ -608,7 +628,15 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
     const auto *in = static_cast<const kp::bq::rec_t<real> *>(Q.in);
     kp::bq::wave_peel_open();
     const uint64_t n_tasks = *Q.n_tasks;
-    for (uint64_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+    // tasks are claimed one at a time from a counter, so the workgroups that
+    // draw light tasks take more of them
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_t = atomicAdd(Q.task_next, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        if (t >= n_tasks)
+            break;
         {
             if (threadIdx.x == 0) {
                 const uint2 tk = Q.tasks[t];
@@ -671,17 +699,37 @@ __global__ void __launch_bounds__(256, WPS) k_bq_pass(kp::kparams P, kp::bq::qpa
 // The peel-offs queued by a pass: one interaction per lane, in queue order,
 // its rays toward the observers one after the other (the interactions a
 // wave queued lie in one block, so the lanes' rays run parallel).
-template <class Pol, int WPS>
+// CLAIM false: the rays in grid-stride order over a grid of many more
+// workgroups than fit at once.  CLAIM true: each wave claims the next 64
+// rays from a counter until none are left, so the waves end together
+// however long their rays are; with several observers a lane's work (one
+// ray per observer) is long and uneven, and that pays for the claims.
+template <class Pol, int WPS, bool CLAIM>
 __global__ void __launch_bounds__(256, WPS) k_bq_peel(kp::kparams P, kp::bq::qparams Q)
 {
     using real = typename Pol::real;
     kp::lane_counters C;
     const uint64_t n = *Q.pr_count < Q.pr_cap ? *Q.pr_count : Q.pr_cap;
     const auto *rays = static_cast<const kp::bq::ray_rec_t<real> *>(Q.pr);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
-        if (rays[k].ib >= 0) // not the unused rest of a wave's chunk
-            C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
+    if constexpr (CLAIM) {
+        const int lane = threadIdx.x & 63;
+        for (;;) {
+            unsigned long long base = 0;
+            if (lane == 0)
+                base = atomicAdd(Q.peel_next, 64ull);
+            base = __shfl(base, 0, 64);
+            if (base >= n)
+                break;
+            const uint64_t k = base + (uint64_t)lane;
+            if (k < n && rays[k].ib >= 0) // not the unused rest of a wave's chunk
+                C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
+        }
+    } else {
+        const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+        for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+            if (rays[k].ib >= 0) // not the unused rest of a wave's chunk
+                C.peel += kp::bq::peel_ray<Pol>(P, rays[k]);
+    }
     flush_counters(P, C);
 }
 
@@ -1007,8 +1055,6 @@ void free_bufs(fwd_bufs &b)
         dfree(b.bq_rec[k]);
         dfree(b.bq_key[k]);
     }
-    dfree(b.bq_order);
-    dfree(b.bq_skey);
     dfree(b.bq_seg);
     dfree(b.bq_toff);
     dfree(b.bq_task);
@@ -1315,15 +1361,11 @@ void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int 
             dfree(fb.bq_rec[k]);
             dfree(fb.bq_key[k]);
         }
-        dfree(fb.bq_order);
-        dfree(fb.bq_skey);
         fb.bq_cap = fb.bq_rec_bytes = 0;
         for (int k = 0; k < 2; ++k) {
             fb.bq_rec[k] = dmalloc<unsigned char>(n * rec_bytes);
             fb.bq_key[k] = dmalloc<uint32_t>(n);
         }
-        fb.bq_order = dmalloc<int32_t>(n);
-        fb.bq_skey = dmalloc<uint32_t>(n);
         fb.bq_cap = n;
         fb.bq_rec_bytes = rec_bytes;
     }
@@ -1358,7 +1400,7 @@ void ensure_bq(fwd_bufs &fb, uint64_t n, size_t rec_bytes, uint32_t n_keys, int 
         fb.bq_ws_cap = wsb;
     }
     if (!fb.bq_cnt)
-        fb.bq_cnt = dmalloc<unsigned long long>(4);
+        fb.bq_cnt = dmalloc<unsigned long long>(6);
 }
 
 // Peel-off ray queue of the block queue: `rays` records of ray_bytes each.
@@ -1415,8 +1457,9 @@ fwd_plan plan_forward(kp_engine *e, fwd_bufs &fb, kp::kparams &P, uint64_t total
     if (P.image && pl.nI > 0) {
         P.img_nrep = pick_replicas(S.img_replicas, pl.nI, pl.nI / (uint64_t)run->n_bands,
                                    pl.blocks, 32ull << 20);
-        if (P.img_nrep > 1)
-            ensure_replicas(fb.img_rep, fb.img_cap, (uint64_t)P.img_nrep * pl.nI, stream);
+        // the kernels add into planar replicas (even just one), folded into
+        // the caller's image after the launch
+        ensure_replicas(fb.img_rep, fb.img_cap, (uint64_t)P.img_nrep * pl.nI, stream);
     }
     // block queue: the bands from queue_from_band on
     pl.n_blocks = e->n_blocks;
@@ -1525,10 +1568,18 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     Q.emit_np = fb.bq_np;
     Q.pr = pl.bq_ray_cap > 0 ? fb.bq_rays : nullptr;
     Q.pr_count = fb.bq_cnt + 2;
+    Q.peel_next = fb.bq_cnt + 5;
     Q.pr_cap = pl.bq_ray_cap;
-    const unsigned peel_blocks = (unsigned)pl.bq_finish_blocks * 2;
+    // one observer: grid-stride order over 32 times the workgroups that fit
+    // (config 2 mixed, 1e8 packets: 420 ms per step, 429 with claims);
+    // several: claims over the workgroups that fit (config 5, three
+    // observers: 1010 ms per 1e8-packet chunk, 1044 in grid-stride order)
+    const bool claim = P.n_incl > 1;
+    const unsigned peel_blocks = (unsigned)(pl.bq_finish_blocks * (claim ? 1 : 32));
+    const auto peel = claim ? k_bq_peel<Pol, kPeelWaves<Pol>, true>
+                            : k_bq_peel<Pol, kPeelWaves<Pol>, false>;
     if (Q.pr)
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);
+        hipLaunchKernelGGL(k_zero_u64x2, dim3(1), dim3(1), 0, stream, Q.pr_count, Q.peel_next);
     const int nqb = P.n_bands - pl.bq_first_band;
     hipLaunchKernelGGL(k_bq_emit_peel<Pol>, dim3((unsigned)((nqb + 63) / 64)), dim3(64), 0,
                        stream, P, Q);
@@ -1543,27 +1594,30 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     int cur = 0;
     for (int pass = 0; pass < pl.bq_passes; ++pass) {
         const int nxt = 1 - cur;
-        kp::sort_order(reinterpret_cast<const int32_t *>(fb.bq_key[cur]), pl.bq_n_slots,
-                       pl.bq_key_bits, fb.bq_order, fb.bq_ws, fb.bq_ws_cap, stream,
-                       fb.bq_cnt + cur);
-        hipLaunchKernelGGL(k_bq_skey, dim3(grid_n(pl.bq_n_slots)), dim3(256), 0, stream,
-                           fb.bq_key[cur], fb.bq_order, fb.bq_cnt + cur, fb.bq_skey);
+        // the sort leaves the permutation and the sorted keys in its
+        // workspace, where this pass reads them
+        const int32_t *order = nullptr;
+        const uint32_t *skey = nullptr;
+        kp::sort_in_place(reinterpret_cast<const int32_t *>(fb.bq_key[cur]), pl.bq_n_slots,
+                          pl.bq_key_bits, fb.bq_ws, fb.bq_ws_cap, stream, fb.bq_cnt + cur,
+                          &order, &skey);
         hipLaunchKernelGGL(k_bq_seg, dim3(grid_n((uint64_t)pl.bq_n_keys + 1)), dim3(256), 0,
-                           stream, fb.bq_skey, fb.bq_cnt + cur, pl.bq_n_keys, fb.bq_seg);
+                           stream, skey, fb.bq_cnt + cur, pl.bq_n_keys, fb.bq_seg);
         hipLaunchKernelGGL(k_bq_tcount, dim3(grid_n(pl.bq_n_keys)), dim3(256), 0, stream,
                            fb.bq_seg, pl.bq_n_keys, pl.bq_chunk, fb.bq_toff);
         hipLaunchKernelGGL(k_bq_tscan, dim3(1), dim3(1024), 0, stream, fb.bq_toff,
-                           pl.bq_n_keys, fb.bq_cnt + 3);
+                           pl.bq_n_keys, fb.bq_cnt + 3, fb.bq_cnt + 4);
         hipLaunchKernelGGL(k_bq_tfill, dim3(grid_n(fb.bq_task_cap)), dim3(256), 0, stream,
                            fb.bq_seg, fb.bq_toff, pl.bq_n_keys, pl.bq_chunk, fb.bq_cnt + 3,
                            fb.bq_task);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, fb.bq_cnt + nxt);
         Q.in = fb.bq_rec[cur];
         Q.in_key = fb.bq_key[cur];
-        Q.order = fb.bq_order;
+        Q.order = order;
         Q.seg = fb.bq_seg;
         Q.tasks = fb.bq_task;
         Q.n_tasks = fb.bq_cnt + 3;
+        Q.task_next = fb.bq_cnt + 4;
         Q.n_in = fb.bq_cnt + cur;
         Q.out = fb.bq_rec[nxt];
         Q.out_key = fb.bq_key[nxt];
@@ -1575,9 +1629,9 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
                            (size_t)P.n3 * (sizeof(kp::bq::tile_t<Pol>) + sizeof(float)),
                            stream, P, Q);
         if (Q.pr) {
-            hipLaunchKernelGGL((k_bq_peel<Pol, kPeelWaves<Pol>>), dim3(peel_blocks), dim3(256),
-                               0, stream, P, Q);
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, Q.pr_count);
+            hipLaunchKernelGGL(peel, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
+            hipLaunchKernelGGL(k_zero_u64x2, dim3(1), dim3(1), 0, stream, Q.pr_count,
+                               Q.peel_next);
         }
         cur = nxt;
     }
@@ -1587,8 +1641,7 @@ void issue_blockq(const fwd_bufs &fb, const kp::kparams &P, const fwd_plan &pl,
     hipLaunchKernelGGL(k_bq_finish<Pol>, dim3((unsigned)pl.bq_finish_blocks), dim3(256), 0,
                        stream, P, Q);
     if (Q.pr)
-        hipLaunchKernelGGL((k_bq_peel<Pol, kPeelWaves<Pol>>), dim3(peel_blocks), dim3(256), 0,
-                           stream, P, Q);
+        hipLaunchKernelGGL(peel, dim3(peel_blocks), dim3(256), 0, stream, P, Q);
 }
 
 // Launches of a planned forward run on `stream`: n_chunks transport kernels
@@ -1601,8 +1654,10 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
     double *J_out = P.J, *I_out = P.image;
     if (P.J && (P.j_nrep > 1 || pl.j_band_major))
         P.J = fb.j_rep;
-    if (P.image && P.img_nrep > 1)
+    if (P.image && pl.nI > 0) {
         P.image = fb.img_rep;
+        P.img_plane = pl.nI / 3;
+    }
     const uint64_t off0 = P.packet_offset;
     for (int k = 0; k < n_chunks; ++k) {
         P.packet_offset = off0 + (uint64_t)k * chunk_stride;
@@ -1657,9 +1712,9 @@ void issue_forward(const kp_schedule &S, const fwd_bufs &fb, kp::kparams P, cons
     else if (J_out && P.j_nrep > 1)
         hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, stream, J_out, fb.j_rep, pl.nJ,
                            P.j_nrep);
-    if (I_out && P.img_nrep > 1)
-        hipLaunchKernelGGL(k_fold, dim3(512), dim3(256), 0, stream, I_out, fb.img_rep, pl.nI,
-                           P.img_nrep);
+    if (I_out && pl.nI > 0)
+        hipLaunchKernelGGL(k_fold_planar, dim3(512), dim3(256), 0, stream, I_out, fb.img_rep,
+                           pl.nI / 3, P.img_nrep);
     KP_HIP(hipGetLastError());
 }
 

@@ -667,10 +667,9 @@ __device__ __forceinline__ void peel_end_stage(const kparams &P, lane_t<Pol> &L,
                 atomicAdd(&kp_lds[k + 1], (double)v[1]);
                 atomicAdd(&kp_lds[k + 2], (double)v[2]);
             } else {
-                double *q = img_base(P) + 3 * px;
-                atomic_add_f64(q + 0, (double)v[0]);
-                atomic_add_f64(q + 1, (double)v[1]);
-                atomic_add_f64(q + 2, (double)v[2]);
+                atomic_add_f64(img_at(P, px, 0), (double)v[0]);
+                atomic_add_f64(img_at(P, px, 1), (double)v[1]);
+                atomic_add_f64(img_at(P, px, 2), (double)v[2]);
             }
         }
     }
@@ -832,10 +831,9 @@ template <class Pol> __device__ void flush_lds_acc(const kparams &P)
             px = f32::pixel(P, b, o, g);
         if (px < 0)
             continue;
-        double *q = img_base(P) + 3 * px;
-        atomic_add_f64(q + 0, v[0]);
-        atomic_add_f64(q + 1, v[1]);
-        atomic_add_f64(q + 2, v[2]);
+        atomic_add_f64(img_at(P, px, 0), v[0]);
+        atomic_add_f64(img_at(P, px, 1), v[1]);
+        atomic_add_f64(img_at(P, px, 2), v[2]);
     }
 }
 

@@ -229,6 +229,26 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *
     return tau;
 }
 
+// Per axis of a direction: whether the face ahead is the upper one, and the
+// moves of the face ahead (+-1) and of the linear cell index (+-1, +-n,
+// +-n^2) when the ray crosses it.  Held as named registers and picked by
+// selects (the axis is a constant wherever the loops are unrolled).
+struct face_steps {
+    int u0, u1, u2, c0, c1, c2;
+    __device__ face_steps(const float *d, int n)
+    {
+        u0 = d[0] > 0.0f ? 1 : 0;
+        u1 = d[1] > 0.0f ? 1 : 0;
+        u2 = d[2] > 0.0f ? 1 : 0;
+        c0 = u0 ? 1 : -1;
+        c1 = u1 ? n : -n;
+        c2 = u2 ? n * n : -(n * n);
+    }
+    __device__ int up(int a) const { return a == 0 ? u0 : (a == 1 ? u1 : u2); }
+    __device__ float sf(int a) const { return up(a) ? 1.0f : -1.0f; }
+    __device__ int sc(int a) const { return a == 0 ? c0 : (a == 1 ? c1 : c2); }
+};
+
 // tau_to_edge with the block changes of a wave's rays taken together: each
 // lane marches to the edge of its block (or its end), and the rays that left
 // their block change it once every ray of the wave has left its block or
@@ -236,6 +256,14 @@ __device__ double tau_to_edge(const kparams &P, const ray_t &from, const float *
 // following blocks together, and the block-change code runs once per block
 // instead of in nearly every step.  Per ray the operations and their order
 // are those of tau_to_edge, so the result is the same.
+//
+// Within a block the step keeps, instead of the integer cell indices, the
+// face ahead on each axis as a float (the value get_dl converts the index
+// to) and the linear cell index, moved by +-1, +-n or +-n^2.  The ray has
+// left the block when a face ahead passed n or 0 (the indices are rebuilt
+// from the faces for change_block).  The lengths, positions and optical
+// depths are computed as in get_dl and cross_in_block, so they are the same
+// numbers.
 __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const float *dir,
                                    float ks, uint64_t &ncross)
 {
@@ -245,24 +273,69 @@ __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const fl
     r.d[2] = dir[2];
     set_scales(P, r);
     double tau = 0.0;
+    // an axis with d = 0 never has the nearest face; with none left the ray
+    // has no face ahead at all (get_dl returns -1)
+    const bool nz0 = r.d[0] != 0.0f, nz1 = r.d[1] != 0.0f, nz2 = r.d[2] != 0.0f;
+    if (!(nz0 | nz1 | nz2))
+        return tau;
+    const int n = P.n_cell;
+    const float nf = (float)n;
+    const face_steps S(r.d, n);
+    const uint32_t cmax = (uint32_t)P.n3 - 1u;
     const float *kb = P.kappa + (size_t)r.ib * P.n3; // the block's kappa_rho
-    float kc = kb[cell_index(P, r)];
+    int c = cell_index(P, r);
+    float kc = kb[c];
+    float ff[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        ff[a] = (float)(r.i[a] + S.up(a));
     for (;;) {
         int ap, st;
         for (;;) {
-            float l;
-            ap = get_dl(r, l);
-            if (ap < 0) {
+            // get_dl: the nearest face ahead, ties to the later axis
+            float t[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                t[a] = fmaxf((ff[a] - r.x[a]) * r.lpc[a], 0.0f);
+            ap = -1;
+            float l = kInf;
+            if (t[0] <= l && nz0) {
+                ap = 0;
+                l = t[0];
+            }
+            if (t[1] <= l && nz1) {
+                ap = 1;
+                l = t[1];
+            }
+            if (t[2] <= l && nz2) {
+                ap = 2;
+                l = t[2];
+            }
+            if (ap < 0) { // no face ahead (get_dl's -1), as in tau_to_edge
                 st = ST_LOST;
                 break;
             }
             const float kap = kc * ks;
-            st = cross_in_block(P, r, ap, l);
+            // cross_in_block: the exit axis onto its face, the others along
+            // the ray
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (a == ap) {
+                    r.x[a] = ff[a];
+                    ff[a] += S.sf(a);
+                    c += S.sc(a);
+                } else {
+                    r.x[a] = fmaf(l, r.cpl[a], r.x[a]);
+                }
+            }
+            const float fhi = fmaxf(fmaxf(ff[0], ff[1]), ff[2]);
+            const float flo = fminf(fminf(ff[0], ff[1]), ff[2]);
+            st = fhi > nf ? ST_BLOCK_HI : (flo < 0.0f ? ST_BLOCK_LO : ST_IN);
             // issued whether or not the ray is still in the block (then the
             // value is not used), so kc is one register loaded one step
             // ahead: a conditional load here made the compiler copy it into
             // place, waiting for the load within the step
-            kc = kb[(uint32_t)cell_index_clamped(P, r)];
+            kc = kb[min((uint32_t)c, cmax)];
             tau += (double)(kap * l);
             ncross += 1;
             if (tau > kTauCut) {
@@ -274,10 +347,17 @@ __device__ double tau_to_edge_sync(const kparams &P, const ray_t &from, const fl
         }
         if (st != ST_BLOCK_LO && st != ST_BLOCK_HI)
             break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            r.i[a] = (int)ff[a] - S.up(a);
         if (change_block(P, r, ap, st == ST_BLOCK_HI) != ST_IN)
             break;
         kb = P.kappa + (size_t)r.ib * P.n3;
-        kc = kb[cell_index(P, r)];
+        c = cell_index(P, r);
+        kc = kb[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            ff[a] = (float)(r.i[a] + S.up(a));
     }
     return tau;
 }
@@ -319,9 +399,9 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     const long pix = pixel(P, band, o, gx);
     if (pix < 0)
         return;
-    atomic_add_f64(img_base(P) + 3 * pix + 0, (double)I);
-    atomic_add_f64(img_base(P) + 3 * pix + 1, (double)Q);
-    atomic_add_f64(img_base(P) + 3 * pix + 2, (double)U);
+    atomic_add_f64(img_at(P, pix, 0), (double)I);
+    atomic_add_f64(img_at(P, pix, 1), (double)Q);
+    atomic_add_f64(img_at(P, pix, 2), (double)U);
 }
 
 __device__ __forceinline__ void phase(int model, float gg, float pmax, float mu, float &z11,

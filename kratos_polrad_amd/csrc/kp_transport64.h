@@ -321,9 +321,9 @@ __device__ __forceinline__ void deposit(const kparams &P, int band, int o, const
     const long pix = pixel(P, band, o, x);
     if (pix < 0)
         return;
-    atomic_add_f64(img_base(P) + 3 * pix + 0, I);
-    atomic_add_f64(img_base(P) + 3 * pix + 1, Q);
-    atomic_add_f64(img_base(P) + 3 * pix + 2, U);
+    atomic_add_f64(img_at(P, pix, 0), I);
+    atomic_add_f64(img_at(P, pix, 1), Q);
+    atomic_add_f64(img_at(P, pix, 2), U);
 }
 
 __device__ __forceinline__ void phase(int model, double gg, double pmax, double mu, double &z11,
