@@ -34,11 +34,11 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 #   persistent kernel: 563.08 GB fetched + 490.39 GB written = 1053.47 GB
 #   against 606.92 GB algorithmic (profiles/k_persist_pol32_config2_amr3.md;
 #   a 5e6-packet dispatch gave 1.735);
-#   block queue, all dispatches of the step: 297.35 + 253.37 = 550.72 GB
+#   block queue, all dispatches of the step: 280.24 + 231.59 = 511.82 GB
 #   (profiles/block_queue_config2_amr3.md): below the algorithmic bytes,
 #   since the J tile takes the FP64 read-modify-writes of most crossings.
 TRAFFIC_PER_ALG_BYTE = {("config2_amr3_r2_lognormal_hg", 1): 1053.47 / 606.92,
-                        ("config2_amr3_r2_lognormal_hg", 2): 550.72 / 606.92}
+                        ("config2_amr3_r2_lognormal_hg", 2): 511.82 / 606.93}
 TRAFFIC_SOURCE = {("config2_amr3_r2_lognormal_hg", 1): (
     "rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic byte of the 1e8-packet "
     "k_persist dispatch, profiles/k_persist_pol32_config2_amr3.md"),
@@ -583,14 +583,13 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0, queue_passe
         tuned = eng.autotune(w.source, run, sample_packets=4_000_000)
         best = None
         cands = [{**tuned, "coherent": 0}, {**tuned, "coherent": 1}]
-        if tuned["kind"] != _abi.KP_SCHED_BLOCKQ:
-            # the queue with its own default knobs (every pass workgroup
-            # resident), and with twice its default passes: the disk's packets
-            # stay alive for many passes, and a 4e6-packet sample
-            # undervalues passes (their fixed cost is a larger share of it)
-            qd = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0, "coherent": 0,
-                  "blocks_per_cu": 4, "queue_passes": 0, "queue_chunk": 0, "queue_yield": 0}
-            cands += [{**qd, "queue_passes": qp} for qp in queue_passes]
+        # the queue with its own default knobs (every pass workgroup
+        # resident) and the listed pass counts, whatever the tuner picked:
+        # the disk's packets stay alive for many passes, and a 4e6-packet
+        # sample undervalues passes (their fixed cost is a larger share of it)
+        qd = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0, "coherent": 0,
+              "blocks_per_cu": 4, "queue_passes": 0, "queue_chunk": 0, "queue_yield": 0}
+        cands += [{**qd, "queue_passes": qp} for qp in queue_passes]
         for sc in cands:
             eng.set_schedule(**sc)
             o = eng.alloc_outputs(run)

@@ -78,7 +78,7 @@ static_assert(sizeof(ray_rec_t<float>) == 64 && sizeof(ray_rec_t<double>) == 128
 struct qparams {
     const void *in;                   // records of this pass
     const uint32_t *in_key;           // their keys
-    const int32_t *order;             // sorted position -> record
+    const int32_t *order;             // sorted position -> record (mixed path: null = the same)
     const uint32_t *seg;              // [n_keys + 1] first sorted position of each key
     const uint2 *tasks;               // (first sorted position, key) of each task
     const unsigned long long *n_tasks; // tasks of this pass
@@ -101,6 +101,12 @@ struct qparams {
     unsigned long long *peel_next;    // next ray the peel kernel hands out (zeroed with pr_count)
     uint64_t pr_cap;
 };
+
+// The record at sorted position q.
+__device__ __forceinline__ int32_t record_at(const qparams &Q, uint64_t q)
+{
+    return Q.order ? Q.order[q] : (int32_t)q;
+}
 
 // A packet between march steps.
 template <class Pol> struct qlane {
