@@ -103,9 +103,9 @@ def parse():
                    help="schedule kind of the main workload: the autotuner's pick screened at "
                         "full size, or that kind with the tuned knobs")
     p.add_argument("--queue-knobs", default="",
-                   help="with --main-kind block_queue: passes,chunk,yield,blocks_per_cu of the "
-                        "queue instead of the tuned ones (for counter runs, under which the "
-                        "tuner's sample timings are slowed)")
+                   help="with --main-kind block_queue: passes,chunk,yield,blocks_per_cu[,"
+                        "j_replicas,img_replicas] of the queue instead of the tuned ones (for "
+                        "profiler runs, under which the tuner's sample timings are slowed)")
     p.add_argument("--skip-adjoint", action="store_true")
     p.add_argument("--adjoint-packets", type=float, default=1e7)
     p.add_argument("--skip-batched", action="store_true")
@@ -644,12 +644,23 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0, queue_passe
     # one path-replay pass: gradient of the total I flux of all images
     wgt = torch.zeros(img32.shape, dtype=torch.float64, device="cuda")
     wgt[..., 0] = 1.0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gr = eng.adjoint(w.source, w.run, wgt)
-    torch.cuda.synchronize()
-    tg = time.perf_counter() - t0
-    res["gradient_pass"] = {"seconds": tg, "packets": chunk,
+    # the replay runs on the persistent kernel, whose event batch the
+    # forward's tuning (on the block queue, which has none) leaves to chance:
+    # both batches are timed (6.2 against 8.0 s per 1e8 packets), the faster
+    # counts
+    tg, gr, eb_g = None, None, None
+    for eb in (8, 16):
+        eng.set_schedule(**{**scheds[_abi.KP_F32], "event_batch": eb})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g_eb = eng.adjoint(w.source, w.run, wgt)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if tg is None or dt < tg:
+            tg, gr, eb_g = dt, g_eb, eb
+        del g_eb
+        progress(f"large: gradient pass, event batch {eb}, {dt:.2f} s")
+    res["gradient_pass"] = {"seconds": tg, "packets": chunk, "event_batch": eb_g,
                             "over_forward_chunk": tg / runs[_abi.KP_F32][0],
                             "d_logrho_norm": float(torch.linalg.vector_norm(gr["d_logrho"])),
                             "jlin": float(gr["jlin"][0])}
@@ -990,13 +1001,23 @@ def main():
                     "queue_yield": 0}
             if dflt != tuned:
                 cands.append(dflt)
+        # and the replica counts of the memory budget, where the tuner moved
+        # off them: the sample cannot tell 16 image replicas from the
+        # budget's 85 on the queue, while at 1e8 mixed packets they took 442
+        # against 419 ms
+        rdf = {**tuned, "j_replicas": 0, "img_replicas": 0}
+        if rdf != tuned:
+            cands.append(rdf)
         if main_kind == "block_queue":
             cands = [c for c in cands if c["kind"] == _abi.KP_SCHED_BLOCKQ]
             if args.queue_knobs:
-                qp, qc, qy, qb = (int(x) for x in args.queue_knobs.split(","))
-                cands = [{**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0,
-                          "queue_passes": qp, "queue_chunk": qc, "queue_yield": qy,
-                          "blocks_per_cu": qb}]
+                kn = [int(x) for x in args.queue_knobs.split(",")]
+                c = {**tuned, "kind": _abi.KP_SCHED_BLOCKQ, "queue_from_band": 0,
+                     "queue_passes": kn[0], "queue_chunk": kn[1], "queue_yield": kn[2],
+                     "blocks_per_cu": kn[3]}
+                if len(kn) >= 6:
+                    c.update(j_replicas=kn[4], img_replicas=kn[5])
+                cands = [c]
         check = []
         for c in cands:
             eng.set_schedule(**c)
@@ -1010,7 +1031,8 @@ def main():
         eng.set_schedule(**cands[int(np.argmin(check))])
         eb_check["mixed" if prec == _abi.KP_F32 else "fp64"] = {
             "candidates": [{k: c[k] for k in ("kind", "event_batch", "blocks_per_cu",
-                                              "queue_passes", "queue_chunk", "queue_yield")}
+                                              "queue_passes", "queue_chunk", "queue_yield",
+                                              "j_replicas", "img_replicas")}
                            for c in cands],
             "ms": [1e3 * c for c in check]}
         for k in range(warmup):
