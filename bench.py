@@ -644,23 +644,13 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0, queue_passe
     # one path-replay pass: gradient of the total I flux of all images
     wgt = torch.zeros(img32.shape, dtype=torch.float64, device="cuda")
     wgt[..., 0] = 1.0
-    # the replay runs on the persistent kernel, whose event batch the
-    # forward's tuning (on the block queue, which has none) leaves to chance:
-    # both batches are timed (6.2 against 8.0 s per 1e8 packets), the faster
-    # counts
-    tg, gr, eb_g = None, None, None
-    for eb in (8, 16):
-        eng.set_schedule(**{**scheds[_abi.KP_F32], "event_batch": eb})
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        g_eb = eng.adjoint(w.source, w.run, wgt)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        if tg is None or dt < tg:
-            tg, gr, eb_g = dt, g_eb, eb
-        del g_eb
-        progress(f"large: gradient pass, event batch {eb}, {dt:.2f} s")
-    res["gradient_pass"] = {"seconds": tg, "packets": chunk, "event_batch": eb_g,
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gr = eng.adjoint(w.source, w.run, wgt)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    progress(f"large: gradient pass {tg:.2f} s")
+    res["gradient_pass"] = {"seconds": tg, "packets": chunk,
                             "over_forward_chunk": tg / runs[_abi.KP_F32][0],
                             "d_logrho_norm": float(torch.linalg.vector_norm(gr["d_logrho"])),
                             "jlin": float(gr["jlin"][0])}
