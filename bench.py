@@ -37,15 +37,24 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 #   block queue, all dispatches of the step: 280.24 + 231.59 = 511.82 GB
 #   (profiles/block_queue_config2_amr3.md): below the algorithmic bytes,
 #   since the J tile takes the FP64 read-modify-writes of most crossings.
+# And on the config-5 disk, one 1e8-packet chunk of the block queue with 64
+# passes: 1464.73 GB fetched + 475.63 GB written = 1940.36 GB against
+# 1710.41 GB algorithmic (profiles/block_queue_config5_disk.md): the packet
+# records of its many passes.
 TRAFFIC_PER_ALG_BYTE = {("config2_amr3_r2_lognormal_hg", 1): 1053.47 / 606.92,
-                        ("config2_amr3_r2_lognormal_hg", 2): 511.82 / 606.93}
+                        ("config2_amr3_r2_lognormal_hg", 2): 511.82 / 606.93,
+                        ("config5_amr5_disk_1e9", 2): 1940.36 / 1710.41}
 TRAFFIC_SOURCE = {("config2_amr3_r2_lognormal_hg", 1): (
     "rocprofv3 FETCH_SIZE + WRITE_SIZE per algorithmic byte of the 1e8-packet "
     "k_persist dispatch, profiles/k_persist_pol32_config2_amr3.md"),
     ("config2_amr3_r2_lognormal_hg", 2): (
     "rocprofv3 FETCH_SIZE + WRITE_SIZE summed over the dispatches of one "
     "1e8-packet block-queue step, per algorithmic byte, "
-    "profiles/block_queue_config2_amr3.md")}
+    "profiles/block_queue_config2_amr3.md"),
+    ("config5_amr5_disk_1e9", 2): (
+    "rocprofv3 FETCH_SIZE + WRITE_SIZE summed over the dispatches of one "
+    "1e8-packet block-queue chunk (64 passes), per algorithmic byte, "
+    "profiles/block_queue_config5_disk.md")}
 # what one timed step launches, per schedule kind (the roofline's "kernel":
 # achieved = algorithmic bytes of the step / HIP-event time of the step)
 KERNELS_OF_KIND = {
@@ -633,8 +642,12 @@ def large(packets=1_000_000_000, chunk=100_000_000, cpu_seconds=0.0, queue_passe
     res["mixed"] = {"seconds": wall, "value": float((cnt[0] + cnt[1]) / wall),
                     "packets_done": float(cnt[7]), "crossings": float(cnt[0] + cnt[1])}
     b_alg = algorithmic_bytes(cnt, len(w.run.incl_deg))
+    tkey = (w.name, scheds[_abi.KP_F32]["kind"])
     res["roofline"] = {"bound": "hbm", "achieved": b_alg / kern / 1e9, "peak": HBM_PEAK / 1e9,
-                       "unit": "GB/s", "frac": b_alg / kern / HBM_PEAK, "traffic": None,
+                       "unit": "GB/s", "frac": b_alg / kern / HBM_PEAK,
+                       "traffic": (TRAFFIC_PER_ALG_BYTE[tkey] * b_alg
+                                   if tkey in TRAFFIC_PER_ALG_BYTE else None),
+                       "traffic_source": TRAFFIC_SOURCE.get(tkey),
                        "alg_bytes": b_alg, "graph_seconds_hip_events": kern,
                        "note": "algorithmic bytes of all chunks over the HIP-event time of "
                                "the graph replay on its stream"}
