@@ -606,8 +606,9 @@ __global__ void __launch_bounds__(256) k_bq_tfill(const uint32_t *seg, const uin
     }
 }
 
-// One pass of the block queue: tasks of `chunk` sorted records; per block of
-// a task the block's kappa_rho and a J tile in LDS, each packet marched to
+// One pass of the block queue: tasks of `chunk` sorted records, which the
+// workgroups claim one at a time; per task (one block) the block's
+// kappa_rho and a J tile in LDS, each packet marched to
 // the block's edge or its end (in the FP64 path by the wave loop pass_block,
 // with the block's face-neighbour entries in LDS too; in the mixed path one
 // record per lane at a time), the tile added to J.
