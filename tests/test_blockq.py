@@ -149,10 +149,16 @@ def test_f64_config1_bit_exact(oracle_lib):
     _close(got["image"], ref["image"], 1e-11)
 
 
-@pytest.mark.parametrize("passes,yld", [(3, 0), (48, 0), (100, 5)])
-def test_f32_equals_persistent_mixed(passes, yld):
+@pytest.mark.parametrize("passes,yld,n_obs", [(3, 0, 2), (48, 0, 2), (100, 5, 2), (20, 0, 1)],
+                         ids=["3_passes", "48_passes", "100_passes_yield_5", "one_observer"])
+def test_f32_equals_persistent_mixed(passes, yld, n_obs):
+    """Two observers: the peel-off kernel's waves claim rays from a counter;
+    one observer: the rays in grid-stride order over a large grid."""
     w = _amr_workload(6000, precision=_abi.KP_F32)
-    print(f"blockq f32 passes={passes} yield={yld}", flush=True)
+    if n_obs == 1:
+        w = configs.Workload(w.name, w.mesh, w.kappa, w.albedo, w.g, w.source,
+                             RunSpec(**{**w.run.__dict__, "incl_deg": (90.0,)}))
+    print(f"blockq f32 passes={passes} yield={yld} observers={n_obs}", flush=True)
     a = _run(w, PERSISTENT)
     b = _run(w, _bq(passes=passes, chunk=512, yld=yld))
     _same_states(b["end_state"], a["end_state"])
