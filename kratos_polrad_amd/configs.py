@@ -85,7 +85,7 @@ def gaussian_random_field(n: int, seed: int, slope: float = -11.0 / 3.0) -> np.n
     nz = kk > 0
     amp[nz] = kk[nz] ** (slope / 2.0)
     noise = rng.standard_normal((n, n, n // 2 + 1)) + 1j * rng.standard_normal((n, n, n // 2 + 1))
-    f = np.fft.irfftn(noise * amp, s=(n, n, n))
+    f = np.fft.irfftn(noise * amp, s=(n, n, n), axes=(0, 1, 2))
     f -= f.mean()
     f /= f.std()
     return f
